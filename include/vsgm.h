@@ -1,0 +1,128 @@
+/*
+ * vsgm.h — C-ABI of the host library (libvsgm.so): graph IO and generators,
+ * queries and the planner, view bins and the pipeline executor.
+ *
+ * It exposes, as plain functions over opaque handles, the C++ host API that
+ * the reference's pm_pipeline drives in-process
+ * (src/app/pattern_matching_pipeline.cu:58-198): Graph, Pattern / Clique /
+ * Graphlet queries, Plan::Optimize, ViewBinManager / ViewBin, and
+ * PipelineExecutor (ctor, Transfer, Match, GetTotalCounts). Functions return
+ * 0 on success and a nonzero vsgm_status otherwise; the message is in
+ * vsgm_error() (per calling thread). Handles are freed with their *_free.
+ */
+#ifndef VSGM_H_
+#define VSGM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vsgm_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vsgm_graph vsgm_graph;
+typedef struct vsgm_query vsgm_query;
+typedef struct vsgm_plan vsgm_plan;
+typedef struct vsgm_view_manager vsgm_view_manager;
+typedef struct vsgm_view_holder vsgm_view_holder;
+typedef struct vsgm_executor vsgm_executor;
+
+const char* vsgm_error(void);
+
+/* ---- graph (src/graph/graph_io.h, src/graph/cpu_graph.h) ---------------- */
+/* GraphIO::ReadCSRBinFile (graph_io.h:57-100). */
+int vsgm_graph_read_bin(const char* path, vsgm_graph** out);
+/* GraphIO::WriteCSRBinFile (graph_io.h:313-339); writes exactly `path`. */
+int vsgm_graph_write_bin(const vsgm_graph* g, const char* path);
+/* Graph from a CSR (copied); lists must be sorted and symmetric. */
+int vsgm_graph_from_csr(uint64_t vertex_count, const uint64_t* row_ptrs, const uint32_t* cols, vsgm_graph** out);
+/* Graph from undirected edge pairs (cleaned: symmetrised, deduplicated, no self loops). */
+int vsgm_graph_from_edges(uint64_t vertex_count, const uint32_t* pairs, uint64_t n_pairs, vsgm_graph** out);
+int vsgm_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads, vsgm_graph** out);
+int vsgm_graph_chung_lu(uint64_t vertex_count, uint64_t draws, double gamma, uint64_t seed, int threads, vsgm_graph** out);
+int vsgm_graph_info(const vsgm_graph* g, uint64_t* vertex_count, uint64_t* entries, uint64_t* max_degree);
+/* Borrowed pointers into the graph's CSR (valid until vsgm_graph_free). */
+int vsgm_graph_csr(const vsgm_graph* g, const uint64_t** row_ptrs, const uint32_t** cols);
+/* Writes "" to msg when the CSR is sorted, loop-free, duplicate-free and symmetric. */
+int vsgm_graph_validate(const vsgm_graph* g, char* msg, size_t len);
+void vsgm_graph_free(vsgm_graph* g);
+
+/* ---- queries (src/query/pattern.h, clique.h, graphlet.h, query.h) ------- */
+int vsgm_query_preset(int preset, vsgm_query** out);  /* Pattern(Pk, false) */
+int vsgm_query_clique(int k, vsgm_query** out);       /* Clique(k) */
+/* Connected pattern from an n x n 0/1 adjacency matrix (Query(n, m, adj), query.h:62-78). */
+int vsgm_query_from_matrix(int n, const uint8_t* adj, vsgm_query** out);
+/* Graphlet(k).GetQueries() (graphlet.h:15-136): *n_out queries written to
+ * out[0..cap); *n_out is set even when cap is too small. */
+int vsgm_query_graphlets(int k, vsgm_query** out, int cap, int* n_out);
+/* n, undirected edges, remapped adjacency matrix and condition matrix
+ * (cond[a*n+b] = operator between a and b, VSGM_NON_EQUAL when unrelated). */
+int vsgm_query_info(const vsgm_query* q, int* n, int* edges, uint8_t* adj, uint8_t* cond);
+void vsgm_query_free(vsgm_query* q);
+
+/* ---- plans (src/query/plan.h) -------------------------------------------- */
+int vsgm_plan_optimize(const vsgm_query* q, vsgm_plan** out); /* Plan(q).Optimize() */
+/* Plan over a caller-chosen connected matching order seq[0..n). */
+int vsgm_plan_from_order(const vsgm_query* q, const uint32_t* seq, int n, vsgm_plan** out);
+int vsgm_plan_info(const vsgm_plan* p, int* hop, uint64_t* root_degree, int* n_ops);
+/* Operation i: op (TraversalOperation numbering of plan.h:14-21) and vertex. */
+int vsgm_plan_op(const vsgm_plan* p, int i, int* op, int* vertex);
+/* Matching order (search sequence) into seq[0..n). */
+int vsgm_plan_order(const vsgm_plan* p, uint32_t* seq, int cap);
+/* Key/value text of the whole plan (the planner parity fixtures' format);
+ * returns the needed size in *needed. */
+int vsgm_plan_serialize(const vsgm_plan* p, char* buf, size_t len, size_t* needed);
+void vsgm_plan_free(vsgm_plan* p);
+
+/* ---- view bins (src/view) ------------------------------------------------ */
+int vsgm_vbmap_write(const char* path, int n_bins, const int* assignment, uint64_t vertex_count);
+int vsgm_view_manager_load(const vsgm_graph* g, int hop, int threads, const char* vbmap, vsgm_view_manager** out);
+int vsgm_view_manager_from_assignment(const vsgm_graph* g, int hop, int threads, int n_bins, const int* assignment,
+                                      vsgm_view_manager** out);
+int vsgm_view_manager_info(const vsgm_view_manager* m, int* n_bins, uint64_t* max_sources);
+int vsgm_view_manager_reorder(vsgm_view_manager* m);
+int vsgm_view_manager_split(vsgm_view_manager* m, int devices, int times);
+void vsgm_view_manager_free(vsgm_view_manager* m);
+/* Pinned holder with room for max_bytes of view columns (ViewBinHolder). */
+int vsgm_view_holder_create(uint64_t vertex_count, uint64_t max_bytes, vsgm_view_holder** out);
+/* ViewBin::Materialize of the manager's bin at pool position `index`. */
+int vsgm_view_materialize(const vsgm_view_manager* m, int index, vsgm_view_holder* h, uint64_t root_degree);
+int vsgm_view_holder_info(const vsgm_view_holder* h, int* bin_id, uint64_t* sources, uint64_t* entries, uint64_t* vertices);
+int vsgm_view_holder_csr(const vsgm_view_holder* h, const uint64_t** row_ptrs, const uint32_t** cols, const uint32_t** sources);
+void vsgm_view_holder_free(vsgm_view_holder* h);
+
+/* ---- executor (src/executor/pipeline_executor.h:20-120) ------------------ */
+typedef struct vsgm_level_stat {
+  uint64_t batches;
+  uint64_t paths;
+  uint64_t work;
+  uint64_t outputs;
+  double ms;
+} vsgm_level_stat;
+
+int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* const* plans, int n_plans, uint64_t max_sources,
+                         uint64_t mem_budget, vsgm_executor** out);
+/* Options: "batch_bytes" (0 = budget / remaining levels), "min_batch_bytes",
+ * "profile" (1 = time every level op). */
+int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value);
+int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h);
+/* Whole graph as one view; sources NULL = every vertex. */
+int vsgm_executor_transfer_graph(vsgm_executor* e, const vsgm_graph* g, const uint32_t* sources, uint64_t n_sources);
+int vsgm_executor_match(vsgm_executor* e, int do_match);
+int vsgm_executor_counts(const vsgm_executor* e, uint64_t* counts, int n);
+int vsgm_executor_match_ms(const vsgm_executor* e, double* ms, int n);
+int vsgm_executor_reset(vsgm_executor* e);
+int vsgm_executor_level_stats(const vsgm_executor* e, int plan, vsgm_level_stat* out, int cap, int* n_out);
+int vsgm_executor_peak_bytes(const vsgm_executor* e, uint64_t* bytes);
+void vsgm_executor_free(vsgm_executor* e);
+
+/* ---- the pm_pipeline driver (src/app/pattern_matching_pipeline.cu:27-201) */
+int vsgm_pm_pipeline(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VSGM_H_ */

@@ -1,0 +1,1182 @@
+// libvsgm_hip.so — gfx950 kernels and the C-ABI declared in include/vsgm_hip.h.
+//
+// Work decomposition. Every level operator of the BFS join is a segmented
+// map over P paths: path p owns a segment [begin_p, begin_p + len_p) of a
+// sorted u32 array (the pivot adjacency list of an intersection, or the
+// path's candidate row), and each element of every segment is one work
+// item. Operators run as
+//   setup   one thread per path: gather the path's bound vertices, pick the
+//           segment and narrow it with the '<' / '>' conditions by binary
+//           search (those conditions are monotone on a sorted list);
+//   scan    exclusive u64 scan of the segment lengths -> element offsets;
+//   map     256-thread workgroups over fixed chunks of 2048 elements. The
+//           chunk's slice of the segment offsets is staged in LDS, so each
+//           element finds its path with an LDS binary search
+//           (load-balance search); the element is verified and either
+//           counted (block reduction + one u64 atomic per chunk) or its
+//           validity bit is written with a wave64 ballot (one u64 mask word
+//           per wavefront) together with a per-chunk valid count;
+//   scatter (compacting ops only) chunk counts are scanned, then each valid
+//           element is written at chunk base + popcount of the valid bits
+//           before it, so outputs keep (path, candidate) order — candidate
+//           rows stay sorted, as the reference requires
+//           (src/gpu/gpu_utils_gpsm_bin_search.h:110-126).
+// All indices over elements and paths are 64-bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "vsgm_hip.h"
+
+namespace {
+
+constexpr int kBlock = 256;               // threads per workgroup (4 wavefronts)
+constexpr int kWave = 64;                 // CDNA wavefront
+constexpr int kItems = 8;                 // elements per thread per chunk
+constexpr int kChunk = kBlock * kItems;   // elements per chunk (multiple of 64)
+constexpr int kWordsPerChunk = kChunk / kWave;
+constexpr int kPathCap = kChunk;          // paths whose offsets fit in LDS per chunk
+constexpr uint64_t kBeginMask = (1ull << 56) - 1;
+constexpr int kMaxGrid = 8192;
+
+enum OpKind { OP_ICOUNT = 0, OP_IWRITE = 1, OP_EXPAND = 2, OP_FILTER = 3, OP_COUNT2 = 4, OP_IPATH = 5 };
+
+__host__ __device__ inline bool is_intersect(int op) { return op == OP_ICOUNT || op == OP_IWRITE || op == OP_IPATH; }
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+__device__ inline uint32_t lower_bound_u32(const uint32_t* a, uint32_t n, uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint32_t upper_bound_u32(const uint32_t* a, uint32_t n, uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// index of the last entry <= key in the non-decreasing a[0..n) (a[0] <= key)
+__device__ inline uint64_t last_le_u64(const uint64_t* a, uint64_t n, uint64_t key) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+
+__device__ inline uint64_t first_ge_u64(const uint64_t* a, uint64_t n, uint64_t key) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint32_t lds_last_le(const uint64_t* a, uint32_t n, uint64_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - 1;
+}
+
+__device__ inline bool cond_holds(uint32_t op, uint32_t c, uint32_t m) {
+  return op == VSGM_LESS_THAN ? (c < m) : (op == VSGM_LARGER_THAN ? (c > m) : (c != m));
+}
+
+__device__ inline uint64_t cand_row(const vsgm_level_args& a, uint32_t u, uint64_t p) {
+  return a.cand_idx[u] ? a.cand_idx[u][p] : a.cand_idx_base[u] + p;
+}
+
+// Number of y in sorted s[0..n) with "x op y" (ThreadCountWithComputedVertex).
+__device__ inline uint64_t count_against(const uint32_t* s, uint32_t n, uint32_t x, uint32_t op) {
+  if (n == 0) return 0;
+  uint32_t pos = lower_bound_u32(s, n, x);
+  uint32_t eq = (pos < n && s[pos] == x) ? 1u : 0u;
+  if (op == VSGM_LESS_THAN) return n - (pos + eq);  // y > x
+  if (op == VSGM_LARGER_THAN) return pos;           // y < x
+  return n - eq;                                    // y != x
+}
+
+// Number of y in sorted s[0..n) with "y op x" (the mirrored side).
+__device__ inline uint64_t count_against_mirror(const uint32_t* s, uint32_t n, uint32_t x, uint32_t op) {
+  if (n == 0) return 0;
+  uint32_t pos = lower_bound_u32(s, n, x);
+  uint32_t eq = (pos < n && s[pos] == x) ? 1u : 0u;
+  if (op == VSGM_LESS_THAN) return pos;               // y < x
+  if (op == VSGM_LARGER_THAN) return n - (pos + eq);  // y > x
+  return n - eq;
+}
+
+// Narrow [lo, hi) of the sorted array arr with the '<' / '>' conditions.
+__device__ inline void narrow_range(const vsgm_level_args& a, const uint32_t* arr, uint64_t p, uint64_t& lo, uint64_t& hi) {
+  for (uint32_t i = 0; i < a.n_cond; ++i) {
+    uint32_t op = a.cond[i].op;
+    if (op != VSGM_LESS_THAN && op != VSGM_LARGER_THAN) continue;
+    if (hi <= lo) break;
+    uint32_t m = a.inst[a.cond[i].operand][p];
+    uint32_t n = (uint32_t)(hi - lo);
+    if (op == VSGM_LESS_THAN) hi = lo + lower_bound_u32(arr + lo, n, m);
+    else lo = lo + upper_bound_u32(arr + lo, n, m);
+  }
+  if (hi < lo) hi = lo;
+}
+
+__device__ inline bool non_equal_ok(const vsgm_level_args& a, uint64_t p, uint32_t c) {
+  for (uint32_t i = 0; i < a.n_cond; ++i)
+    if (a.cond[i].op == VSGM_NON_EQUAL && a.inst[a.cond[i].operand][p] == c) return false;
+  return true;
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// exclusive scan across the workgroup; s_tmp needs kBlock/kWave entries
+__device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t* s_tmp, uint64_t& total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  uint64_t x = v;
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint64_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == kWave - 1) s_tmp[wave] = x;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    uint64_t t = s_tmp[w];
+    if (w < wave) before += t;
+    all += t;
+  }
+  __syncthreads();
+  total = all;
+  return before + x - v;
+}
+
+__device__ inline uint64_t block_sum(uint64_t v, uint64_t* s_tmp) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  if (lane == 0) s_tmp[wave] = v;
+  __syncthreads();
+  uint64_t all = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) all += s_tmp[w];
+  __syncthreads();
+  return all;
+}
+
+inline unsigned grid_for(uint64_t units) {
+  uint64_t g = units == 0 ? 1 : units;
+  return (unsigned)std::min<uint64_t>(g, kMaxGrid);
+}
+
+// ---------------------------------------------------------------------------
+// primitives
+
+__global__ __launch_bounds__(kBlock) void k_tile_sum(const uint64_t* __restrict__ in, uint64_t n, uint64_t* __restrict__ partial,
+                                                     uint64_t n_tiles) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    uint64_t base = t * kChunk, s = 0;
+    for (int i = 0; i < kItems; ++i) {
+      uint64_t e = base + (uint64_t)i * kBlock + threadIdx.x;
+      if (e < n) s += in[e];
+    }
+    s = block_sum(s, s_tmp);
+    if (threadIdx.x == 0) partial[t] = s;
+  }
+}
+
+// Exclusive scan of one tile per workgroup, offset by tile_prefix[t];
+// the workgroup owning the last tile writes out[n] = total.
+__global__ __launch_bounds__(kBlock) void k_tile_scan(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t n_tiles,
+                                                      const uint64_t* __restrict__ tile_prefix) {
+  __shared__ uint64_t s_data[kChunk];
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    uint64_t base = t * kChunk;
+    for (int i = 0; i < kItems; ++i) {
+      uint64_t e = base + (uint64_t)i * kBlock + threadIdx.x;
+      s_data[i * kBlock + threadIdx.x] = e < n ? in[e] : 0;
+    }
+    __syncthreads();
+    uint64_t local[kItems];
+    uint64_t sum = 0;
+    for (int i = 0; i < kItems; ++i) {
+      local[i] = s_data[threadIdx.x * kItems + i];
+      sum += local[i];
+    }
+    uint64_t total;
+    uint64_t run = block_excl_scan(sum, s_tmp, total);
+    uint64_t pre = tile_prefix ? tile_prefix[t] : 0;
+    run += pre;
+    for (int i = 0; i < kItems; ++i) {
+      s_data[threadIdx.x * kItems + i] = run;
+      run += local[i];
+    }
+    __syncthreads();
+    for (int i = 0; i < kItems; ++i) {
+      uint64_t e = base + (uint64_t)i * kBlock + threadIdx.x;
+      if (e < n) out[e] = s_data[i * kBlock + threadIdx.x];
+    }
+    if (t == n_tiles - 1 && threadIdx.x == 0) out[n] = pre + total;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce(const uint64_t* __restrict__ in, uint64_t n, unsigned long long* total) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  uint64_t s = 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (uint64_t)gridDim.x * kBlock) s += in[e];
+  s = block_sum(s, s_tmp);
+  if (threadIdx.x == 0 && s) atomicAdd(total, (unsigned long long)s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(const uint64_t* __restrict__ idx, uint64_t n, const T* __restrict__ src,
+                                                   T* __restrict__ dst) {
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (uint64_t)gridDim.x * kBlock) dst[e] = src[idx[e]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(uint64_t* d, uint64_t n, uint64_t base) {
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (uint64_t)gridDim.x * kBlock) d[e] = base + e;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(uint64_t* d, uint64_t n, uint64_t v) {
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (uint64_t)gridDim.x * kBlock) d[e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// level operators
+
+// Per-path segment: begin (low 56 bits, with the pivot position or the
+// count side in the top byte) and length.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* __restrict__ seg_begin, uint64_t* __restrict__ seg_len) {
+  const uint64_t P = a.path_num;
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kBlock) {
+    uint64_t lo = 0, hi = 0, tag = 0;
+    if (is_intersect(OP)) {
+      uint64_t best_deg = 0;
+      for (uint32_t i = 0; i < a.n_back; ++i) {
+        uint32_t v = a.inst[a.back[i]][p];
+        uint64_t b = a.row_ptrs[v], e = a.row_ptrs[v + 1];
+        if (i == 0 || e - b < best_deg) {  // first minimum wins (ThreadChoosePivotLevel)
+          best_deg = e - b;
+          lo = b;
+          hi = e;
+          tag = i;
+        }
+      }
+      narrow_range(a, a.cols, p, lo, hi);
+    } else if (OP == OP_EXPAND || OP == OP_FILTER) {
+      uint64_t r = cand_row(a, a.target, p);
+      lo = a.cand_off[a.target][r];
+      hi = a.cand_off[a.target][r + 1];
+      narrow_range(a, a.cand[a.target], p, lo, hi);
+    } else {  // OP_COUNT2
+      uint64_t r0 = cand_row(a, a.target, p), r1 = cand_row(a, (uint32_t)a.other, p);
+      uint64_t b0 = a.cand_off[a.target][r0], n0 = a.cand_off[a.target][r0 + 1] - b0;
+      uint64_t b1 = a.cand_off[a.other][r1], n1 = a.cand_off[a.other][r1 + 1] - b1;
+      if (n0 <= n1) {
+        lo = b0;
+        hi = b0 + n0;
+        tag = 0;
+      } else {
+        lo = b1;
+        hi = b1 + n1;
+        tag = 1;
+      }
+    }
+    seg_begin[p] = lo | (tag << 56);
+    seg_len[p] = hi - lo;
+  }
+}
+
+struct ExpandCols {
+  uint32_t n_mat;
+  uint32_t mat[VSGM_MAX_QV];
+  uint32_t n_cu;
+  uint32_t cu[VSGM_MAX_QV];
+  uint32_t* out_inst[VSGM_MAX_QV];
+  uint64_t* out_idx[VSGM_MAX_QV];
+};
+
+template <int OP>
+__device__ inline const uint32_t* seg_source(const vsgm_level_args& a, uint64_t tag) {
+  if (is_intersect(OP)) return a.cols;
+  if (OP == OP_COUNT2) return tag ? a.cand[a.other] : a.cand[a.target];
+  return a.cand[a.target];
+}
+
+// Verify element c of path p. Returns its contribution (count ops) and sets
+// valid.
+template <int OP>
+__device__ inline uint64_t eval_element(const vsgm_level_args& a, uint64_t p, uint64_t tag, uint32_t c, bool& valid) {
+  if (is_intersect(OP)) {
+    bool ok = true;
+    for (uint32_t i = 0; i < a.n_back && ok; ++i) {
+      if (i == tag) continue;
+      uint32_t v = a.inst[a.back[i]][p];
+      uint64_t b = a.row_ptrs[v];
+      uint32_t n = (uint32_t)(a.row_ptrs[v + 1] - b);
+      uint32_t pos = lower_bound_u32(a.cols + b, n, c);
+      ok = pos < n && a.cols[b + pos] == c;
+    }
+    ok = ok && non_equal_ok(a, p, c);
+    valid = ok;
+    if (!ok) return 0;
+    if (OP == OP_ICOUNT && a.other >= 0) {
+      uint64_t r = cand_row(a, (uint32_t)a.other, p);
+      uint64_t b = a.cand_off[a.other][r];
+      uint32_t n = (uint32_t)(a.cand_off[a.other][r + 1] - b);
+      return count_against(a.cand[a.other] + b, n, c, a.other_op);
+    }
+    return 1;
+  } else if (OP == OP_EXPAND || OP == OP_FILTER) {
+    valid = non_equal_ok(a, p, c);
+    return valid ? 1 : 0;
+  } else {  // OP_COUNT2: c comes from the smaller side
+    valid = true;
+    if (tag == 0) {
+      uint64_t r = cand_row(a, (uint32_t)a.other, p);
+      uint64_t b = a.cand_off[a.other][r];
+      uint32_t n = (uint32_t)(a.cand_off[a.other][r + 1] - b);
+      return count_against(a.cand[a.other] + b, n, c, a.other_op);
+    } else {
+      uint64_t r = cand_row(a, a.target, p);
+      uint64_t b = a.cand_off[a.target][r];
+      uint32_t n = (uint32_t)(a.cand_off[a.target][r + 1] - b);
+      return count_against_mirror(a.cand[a.target] + b, n, c, a.other_op);
+    }
+  }
+}
+
+// Chunk bookkeeping shared by the map and scatter kernels: the paths whose
+// segments overlap elements [e0, e1) and (if they fit) their offsets in LDS.
+struct ChunkPaths {
+  uint64_t first;  // path of element e0
+  uint64_t last;   // path of element e1-1
+  uint32_t staged; // number of offsets staged in LDS (0: search global memory)
+};
+
+__device__ inline void stage_chunk(const uint64_t* seg_off, uint64_t P, uint64_t e0, uint64_t e1, ChunkPaths& cp, uint64_t* s_off,
+                                   ChunkPaths* s_cp) {
+  if (threadIdx.x == 0) {
+    ChunkPaths c;
+    c.first = last_le_u64(seg_off, P + 1, e0);
+    c.last = last_le_u64(seg_off, P + 1, e1 - 1);
+    uint64_t span = c.last - c.first + 2;  // offsets first..last+1
+    c.staged = span <= (uint64_t)kPathCap + 1 ? (uint32_t)span : 0;
+    *s_cp = c;
+  }
+  __syncthreads();
+  cp = *s_cp;
+  for (uint32_t i = threadIdx.x; i < cp.staged; i += kBlock) s_off[i] = seg_off[cp.first + i];
+  __syncthreads();
+}
+
+__device__ inline uint64_t path_of(uint64_t e, const ChunkPaths& cp, const uint64_t* s_off, const uint64_t* seg_off, uint64_t& start) {
+  if (cp.staged) {
+    uint32_t i = lds_last_le(s_off, cp.staged, e);
+    start = s_off[i];
+    return cp.first + i;
+  }
+  uint64_t i = last_le_u64(seg_off + cp.first, cp.last - cp.first + 2, e);
+  start = seg_off[cp.first + i];
+  return cp.first + i;
+}
+
+// Map: count (OP_ICOUNT, OP_COUNT2, OP_IPATH) or mark valid elements
+// (OP_IWRITE, OP_EXPAND, OP_FILTER).
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_t* __restrict__ seg_begin, const uint64_t* __restrict__ seg_off,
+                                                uint64_t W, uint64_t n_chunks, uint64_t* __restrict__ mask, uint64_t* __restrict__ chunk_cnt,
+                                                unsigned long long* __restrict__ total, unsigned long long* __restrict__ path_counts) {
+  __shared__ uint64_t s_off[kPathCap + 1];
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ ChunkPaths s_cp;
+  constexpr bool kCompact = (OP == OP_IWRITE || OP == OP_EXPAND || OP == OP_FILTER);
+  const uint64_t P = a.path_num;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  uint64_t acc = 0;
+  for (uint64_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kChunk;
+    const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
+    ChunkPaths cp;
+    stage_chunk(seg_off, P, e0, e1, cp, s_off, &s_cp);
+    uint64_t valid_count = 0;
+    for (int it = 0; it < kItems; ++it) {
+      const uint64_t e = e0 + (uint64_t)it * kBlock + threadIdx.x;
+      bool valid = false;
+      if (e < e1) {
+        uint64_t start;
+        uint64_t p = path_of(e, cp, s_off, seg_off, start);
+        uint64_t packed = seg_begin[p];
+        uint64_t tag = packed >> 56;
+        uint32_t c = seg_source<OP>(a, tag)[(packed & kBeginMask) + (e - start)];
+        uint64_t contrib = eval_element<OP>(a, p, tag, c, valid);
+        if (OP == OP_IPATH) {
+          if (valid) atomicAdd(path_counts + p, 1ull);
+        } else if (!kCompact) {
+          acc += contrib;
+        }
+      }
+      if (kCompact) {
+        uint64_t bits = __ballot(valid);
+        uint64_t word_first = e0 + (uint64_t)it * kBlock + (uint64_t)wave * kWave;
+        if (lane == 0 && word_first < W) mask[word_first / kWave] = bits;
+        valid_count += valid ? 1 : 0;
+      }
+    }
+    if (kCompact) {
+      uint64_t s = block_sum(valid_count, s_tmp);
+      if (threadIdx.x == 0) chunk_cnt[chunk] = s;
+    }
+    __syncthreads();
+  }
+  if (!kCompact && OP != OP_IPATH) {
+    uint64_t s = block_sum(acc, s_tmp);
+    if (threadIdx.x == 0 && s) atomicAdd(total, (unsigned long long)s);
+  }
+}
+
+// Scatter the valid elements of the compacting ops; chunk_off is the
+// exclusive scan of the chunk counts (n_chunks + 1 entries).
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uint64_t* __restrict__ seg_begin,
+                                                    const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t n_chunks,
+                                                    const uint64_t* __restrict__ mask, const uint64_t* __restrict__ chunk_off,
+                                                    uint32_t* __restrict__ out_vals, uint64_t* __restrict__ out_off, ExpandCols cols) {
+  __shared__ uint64_t s_off[kPathCap + 1];
+  __shared__ uint64_t s_word[kWordsPerChunk + 1];
+  __shared__ uint64_t s_wpre[kWordsPerChunk + 1];
+  __shared__ ChunkPaths s_cp;
+  const uint64_t P = a.path_num;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (uint64_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kChunk;
+    const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
+    const uint32_t n_words = (uint32_t)((e1 - e0 + kWave - 1) / kWave);
+    const uint64_t base = chunk_off[chunk];
+    ChunkPaths cp;
+    if (threadIdx.x <= kWordsPerChunk) s_word[threadIdx.x] = threadIdx.x < n_words ? mask[e0 / kWave + threadIdx.x] : 0;
+    stage_chunk(seg_off, P, e0, e1, cp, s_off, &s_cp);  // contains the barriers
+    if (threadIdx.x == 0) {
+      uint64_t run = 0;
+      for (uint32_t w = 0; w <= kWordsPerChunk; ++w) {
+        s_wpre[w] = run;
+        run += __popcll(s_word[w]);
+      }
+    }
+    __syncthreads();
+    for (int it = 0; it < kItems; ++it) {
+      const uint64_t e = e0 + (uint64_t)it * kBlock + threadIdx.x;
+      if (e >= e1) continue;
+      const uint32_t w = (uint32_t)((e - e0) / kWave);
+      const uint64_t word = s_word[w];
+      if (!((word >> lane) & 1ull)) continue;
+      const uint64_t pos = base + s_wpre[w] + __popcll(word & ((1ull << lane) - 1ull));
+      uint64_t start;
+      uint64_t p = path_of(e, cp, s_off, seg_off, start);
+      uint64_t packed = seg_begin[p];
+      uint32_t c = seg_source<OP>(a, packed >> 56)[(packed & kBeginMask) + (e - start)];
+      if (OP == OP_EXPAND) {
+        cols.out_inst[a.target][pos] = c;
+        for (uint32_t i = 0; i < cols.n_mat; ++i) cols.out_inst[cols.mat[i]][pos] = a.inst[cols.mat[i]][p];
+        for (uint32_t i = 0; i < cols.n_cu; ++i) cols.out_idx[cols.cu[i]][pos] = cand_row(a, cols.cu[i], p);
+      } else {
+        out_vals[pos] = c;
+      }
+    }
+    if (OP != OP_EXPAND) {
+      // new row offsets of the paths whose segment starts in this chunk
+      const bool last_chunk = (chunk == n_chunks - 1);
+      const uint64_t p_lo = first_ge_u64(seg_off, P + 1, e0);
+      const uint64_t p_hi = last_chunk ? P + 1 : first_ge_u64(seg_off, P + 1, e1);
+      for (uint64_t p = p_lo + threadIdx.x; p < p_hi; p += kBlock) {
+        uint64_t rel = seg_off[p] - e0;
+        uint32_t w = (uint32_t)(rel / kWave);
+        uint32_t bit = (uint32_t)(rel % kWave);
+        uint64_t word = w < n_words ? s_word[w] : 0;
+        out_off[p] = base + s_wpre[w] + __popcll(word & ((1ull << bit) - 1ull));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_count1(vsgm_level_args a, unsigned long long* total) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  uint64_t s = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.path_num; p += (uint64_t)gridDim.x * kBlock) {
+    uint64_t r = cand_row(a, a.target, p);
+    s += a.cand_off[a.target][r + 1] - a.cand_off[a.target][r];
+  }
+  s = block_sum(s, s_tmp);
+  if (threadIdx.x == 0 && s) atomicAdd(total, (unsigned long long)s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_children(vsgm_level_args a, int mode, uint64_t* __restrict__ out) {
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < a.path_num; p += (uint64_t)gridDim.x * kBlock) {
+    uint64_t v = 1;
+    if (mode == 0) {
+      uint64_t best = 0;
+      for (uint32_t i = 0; i < a.n_back; ++i) {
+        uint32_t x = a.inst[a.back[i]][p];
+        uint64_t d = a.row_ptrs[x + 1] - a.row_ptrs[x];
+        if (i == 0 || d < best) best = d;
+      }
+      v = best;
+    } else if (mode == 1) {
+      uint64_t r = cand_row(a, a.target, p);
+      v = a.cand_off[a.target][r + 1] - a.cand_off[a.target][r];
+    }
+    out[p] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cost(const uint64_t* __restrict__ children, uint64_t n, uint64_t pf, uint64_t cf,
+                                                 uint64_t* __restrict__ cost) {
+  for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (uint64_t)gridDim.x * kBlock) cost[e] = children[e] * cf + pf;
+}
+
+// ends[k] = number of parents whose cost prefix stays <= (k+1) * batch_bytes
+__global__ __launch_bounds__(kBlock) void k_split(const uint64_t* __restrict__ prefix, uint64_t n, uint64_t batch_bytes, uint64_t nb,
+                                                  uint64_t* __restrict__ ends) {
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < nb; k += (uint64_t)gridDim.x * kBlock) {
+    uint64_t target = (k + 1) * batch_bytes;
+    ends[k] = last_le_u64(prefix, n + 1, target);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context and pool
+
+struct vsgm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start[VSGM_TIMER_SLOTS] = {};
+  hipEvent_t ev_stop[VSGM_TIMER_SLOTS] = {};
+  uint64_t budget = 0;
+  uint64_t live = 0;
+  uint64_t peak = 0;
+  std::multimap<uint64_t, void*> cached;  // size class -> free block
+  std::map<void*, uint64_t> blocks;       // live block -> size class
+  uint64_t cached_bytes = 0;
+  uint64_t* h_pinned = nullptr;           // small pinned staging for D2H of totals
+  std::string err;
+  vsgm_op_stats stats{0, 0, 0};
+};
+
+namespace {
+
+int fail(vsgm_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                              \
+  do {                                                                                                  \
+    hipError_t _e = (call);                                                                             \
+    if (_e != hipSuccess) return fail(ctx, VSGM_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, __LINE__); \
+  } while (0)
+
+#define VSGM_TRY(call)          \
+  do {                          \
+    int _s = (call);            \
+    if (_s != VSGM_OK) return _s; \
+  } while (0)
+
+uint64_t size_class(uint64_t bytes) {
+  if (bytes == 0) bytes = 1;
+  if (bytes <= (1ull << 20)) return (bytes + 511) & ~511ull;
+  // quarter steps between powers of two: at most 25% slack
+  int lg = 63 - __builtin_clzll(bytes);
+  uint64_t step = (1ull << lg) / 4;
+  return (bytes + step - 1) / step * step;
+}
+
+int pool_alloc(vsgm_ctx* ctx, uint64_t bytes, void** out) {
+  uint64_t cls = size_class(bytes);
+  if (ctx->live + cls > ctx->budget)
+    return fail(ctx, VSGM_ERR_BUDGET, "allocation of %llu bytes exceeds the budget (%llu live of %llu)", (unsigned long long)cls,
+                (unsigned long long)ctx->live, (unsigned long long)ctx->budget);
+  auto it = ctx->cached.find(cls);
+  void* p = nullptr;
+  if (it != ctx->cached.end()) {
+    p = it->second;
+    ctx->cached.erase(it);
+    ctx->cached_bytes -= cls;
+  } else {
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      // give cached blocks back and retry once
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      for (auto& kv : ctx->cached) (void)hipFree(kv.second);
+      ctx->cached.clear();
+      ctx->cached_bytes = 0;
+      e = hipMalloc(&p, cls);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, VSGM_ERR_OOM, "hipMalloc(%llu) failed: %s", (unsigned long long)cls, hipGetErrorString(e));
+      }
+    }
+  }
+  ctx->blocks[p] = cls;
+  ctx->live += cls;
+  ctx->peak = std::max(ctx->peak, ctx->live);
+  *out = p;
+  return VSGM_OK;
+}
+
+int pool_free(vsgm_ctx* ctx, void* p) {
+  if (!p) return VSGM_OK;
+  auto it = ctx->blocks.find(p);
+  if (it == ctx->blocks.end()) return fail(ctx, VSGM_ERR_ARG, "vsgm_free of a pointer the pool does not own");
+  uint64_t cls = it->second;
+  ctx->blocks.erase(it);
+  ctx->live -= cls;
+  // Blocks are reused only by later work on the same stream, so no sync.
+  ctx->cached.emplace(cls, p);
+  ctx->cached_bytes += cls;
+  return VSGM_OK;
+}
+
+// RAII holder of pool temporaries inside one entry point.
+struct Temps {
+  vsgm_ctx* ctx;
+  std::vector<void*> ptrs;
+  explicit Temps(vsgm_ctx* c) : ctx(c) {}
+  ~Temps() {
+    for (void* p : ptrs) pool_free(ctx, p);
+  }
+  template <typename T>
+  int get(uint64_t n, T** out) {
+    void* p = nullptr;
+    int s = pool_alloc(ctx, std::max<uint64_t>(n, 1) * sizeof(T), &p);
+    if (s != VSGM_OK) return s;
+    ptrs.push_back(p);
+    *out = (T*)p;
+    return VSGM_OK;
+  }
+};
+
+int read_u64(vsgm_ctx* ctx, const uint64_t* d, uint64_t* h) {
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *h = ctx->h_pinned[0];
+  return VSGM_OK;
+}
+
+int launch_check(vsgm_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, VSGM_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return VSGM_OK;
+}
+
+int scan_impl(vsgm_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t n) {
+  uint64_t n_tiles = (n + kChunk - 1) / kChunk;
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, out, 1, 0);
+    return launch_check(ctx, "k_fill");
+  }
+  if (n_tiles == 1) {
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kBlock), 0, ctx->stream, in, out, n, n_tiles, (const uint64_t*)nullptr);
+    return launch_check(ctx, "k_tile_scan");
+  }
+  Temps t(ctx);
+  uint64_t* partial;
+  VSGM_TRY(t.get(n_tiles + 1, &partial));
+  hipLaunchKernelGGL(k_tile_sum, dim3(grid_for(n_tiles)), dim3(kBlock), 0, ctx->stream, in, n, partial, n_tiles);
+  VSGM_TRY(launch_check(ctx, "k_tile_sum"));
+  VSGM_TRY(scan_impl(ctx, partial, partial, n_tiles));
+  hipLaunchKernelGGL(k_tile_scan, dim3(grid_for(n_tiles)), dim3(kBlock), 0, ctx->stream, in, out, n, n_tiles, (const uint64_t*)partial);
+  return launch_check(ctx, "k_tile_scan");
+}
+
+int check_args(vsgm_ctx* ctx, const vsgm_level_args* a) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (!a) return fail(ctx, VSGM_ERR_ARG, "null level args");
+  if (a->n_back > VSGM_MAX_QV || a->n_cond > VSGM_MAX_QV || a->target >= VSGM_MAX_QV) return fail(ctx, VSGM_ERR_ARG, "level args out of range");
+  if (a->other >= VSGM_MAX_QV) return fail(ctx, VSGM_ERR_ARG, "other vertex out of range");
+  for (uint32_t i = 0; i < a->n_cond; ++i)
+    if (a->cond[i].operand >= VSGM_MAX_QV || !a->inst[a->cond[i].operand] || a->cond[i].op > VSGM_NON_EQUAL)
+      return fail(ctx, VSGM_ERR_ARG, "condition %u names an unmaterialized vertex", i);
+  return VSGM_OK;
+}
+
+int check_intersect(vsgm_ctx* ctx, const vsgm_level_args* a) {
+  if (a->n_back == 0) return fail(ctx, VSGM_ERR_ARG, "intersection without backward neighbours");
+  if (!a->row_ptrs || !a->cols) return fail(ctx, VSGM_ERR_ARG, "intersection without a view");
+  for (uint32_t i = 0; i < a->n_back; ++i)
+    if (a->back[i] >= VSGM_MAX_QV || !a->inst[a->back[i]]) return fail(ctx, VSGM_ERR_ARG, "backward vertex %u is not materialized", i);
+  return VSGM_OK;
+}
+
+int check_cand(vsgm_ctx* ctx, const vsgm_level_args* a, uint32_t u) {
+  if (u >= VSGM_MAX_QV || !a->cand_off[u] || !a->cand[u]) return fail(ctx, VSGM_ERR_ARG, "vertex %u has no candidate set", u);
+  return VSGM_OK;
+}
+
+// Shared driver: setup + scan (+ map) of one level operator.
+template <int OP>
+int run_setup_scan(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t** seg_begin, uint64_t** seg_off, uint64_t* W) {
+  const uint64_t P = a.path_num;
+  VSGM_TRY(t.get(P, seg_begin));
+  VSGM_TRY(t.get(P + 1, seg_off));
+  hipLaunchKernelGGL(k_setup<OP>, dim3(grid_for((P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, a, *seg_begin, *seg_off);
+  VSGM_TRY(launch_check(ctx, "k_setup"));
+  VSGM_TRY(scan_impl(ctx, *seg_off, *seg_off, P));
+  VSGM_TRY(read_u64(ctx, *seg_off + P, W));
+  return VSGM_OK;
+}
+
+template <int OP>
+int run_count(vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t* d_total, uint64_t* d_path_counts) {
+  ctx->stats = {a.path_num, 0, 0};
+  if (a.path_num == 0) return VSGM_OK;
+  Temps t(ctx);
+  uint64_t *seg_begin, *seg_off, W;
+  VSGM_TRY(run_setup_scan<OP>(ctx, a, t, &seg_begin, &seg_off, &W));
+  ctx->stats.work = W;
+  if (W == 0) return VSGM_OK;
+  uint64_t n_chunks = (W + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks,
+                     (uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)d_total, (unsigned long long*)d_path_counts);
+  return launch_check(ctx, "k_map(count)");
+}
+
+// Compacting ops: returns the mask-derived chunk offsets and output count.
+template <int OP>
+int run_compact(vsgm_ctx* ctx, const vsgm_level_args& a, uint32_t* vals_or_null, uint64_t* out_off_or_null, ExpandCols cols,
+                Temps& t, uint64_t W, uint64_t* seg_begin, uint64_t* seg_off, uint64_t* mask, uint64_t* chunk_off) {
+  uint64_t n_chunks = (W + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(k_scatter<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask,
+                     chunk_off, vals_or_null, out_off_or_null, cols);
+  return launch_check(ctx, "k_scatter");
+}
+
+template <int OP>
+int run_mark(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t W, uint64_t* seg_begin, uint64_t* seg_off, uint64_t** mask,
+             uint64_t** chunk_off, uint64_t* count) {
+  uint64_t n_chunks = (W + kChunk - 1) / kChunk;
+  VSGM_TRY(t.get((W + kWave - 1) / kWave, mask));
+  VSGM_TRY(t.get(n_chunks + 1, chunk_off));
+  hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, *mask,
+                     *chunk_off, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+  VSGM_TRY(launch_check(ctx, "k_map(mark)"));
+  VSGM_TRY(scan_impl(ctx, *chunk_off, *chunk_off, n_chunks));
+  return read_u64(ctx, *chunk_off + n_chunks, count);
+}
+
+template <int OP>
+int run_csr(vsgm_ctx* ctx, const vsgm_level_args& a, vsgm_csr_out* out) {
+  const uint64_t P = a.path_num;
+  ctx->stats = {P, 0, 0};
+  out->offsets = nullptr;
+  out->values = nullptr;
+  out->count = 0;
+  uint64_t* offsets = nullptr;
+  VSGM_TRY(pool_alloc(ctx, (P + 1) * sizeof(uint64_t), (void**)&offsets));
+  Temps t(ctx);
+  uint64_t *seg_begin = nullptr, *seg_off = nullptr, W = 0;
+  int s = P ? run_setup_scan<OP>(ctx, a, t, &seg_begin, &seg_off, &W) : VSGM_OK;
+  if (s != VSGM_OK) {
+    pool_free(ctx, offsets);
+    return s;
+  }
+  ctx->stats.work = W;
+  uint32_t* values = nullptr;
+  uint64_t count = 0;
+  if (W == 0) {
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((P + 1 + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, offsets, P + 1, 0);
+    s = launch_check(ctx, "k_fill");
+    if (s == VSGM_OK) s = pool_alloc(ctx, sizeof(uint32_t), (void**)&values);
+  } else {
+    uint64_t *mask, *chunk_off;
+    s = run_mark<OP>(ctx, a, t, W, seg_begin, seg_off, &mask, &chunk_off, &count);
+    if (s == VSGM_OK) s = pool_alloc(ctx, std::max<uint64_t>(count, 1) * sizeof(uint32_t), (void**)&values);
+    if (s == VSGM_OK) s = run_compact<OP>(ctx, a, values, offsets, ExpandCols{}, t, W, seg_begin, seg_off, mask, chunk_off);
+  }
+  if (s != VSGM_OK) {
+    pool_free(ctx, offsets);
+    pool_free(ctx, values);
+    return s;
+  }
+  ctx->stats.outputs = count;
+  out->offsets = offsets;
+  out->values = values;
+  out->count = count;
+  return VSGM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+
+extern "C" {
+
+int vsgm_device_count(int* count) {
+  if (!count) return VSGM_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  *count = n;
+  return VSGM_OK;
+}
+
+int vsgm_ctx_create(int device, uint64_t mem_budget, vsgm_ctx** out) {
+  if (!out) return VSGM_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return VSGM_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= n) return VSGM_ERR_ARG;
+  vsgm_ctx* ctx = new vsgm_ctx();
+  ctx->device = device;
+  auto bail = [&](int code) {
+    delete ctx;
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(VSGM_ERR_HIP);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(VSGM_ERR_HIP);
+  for (int i = 0; i < VSGM_TIMER_SLOTS; ++i)
+    if (hipEventCreate(&ctx->ev_start[i]) != hipSuccess || hipEventCreate(&ctx->ev_stop[i]) != hipSuccess) return bail(VSGM_ERR_HIP);
+  if (hipHostMalloc((void**)&ctx->h_pinned, 64, hipHostMallocDefault) != hipSuccess) return bail(VSGM_ERR_HIP);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bail(VSGM_ERR_HIP);
+  ctx->budget = mem_budget ? mem_budget : (uint64_t)(0.9 * (double)free_b);
+  *out = ctx;
+  return VSGM_OK;
+}
+
+int vsgm_ctx_destroy(vsgm_ctx* ctx) {
+  if (!ctx) return VSGM_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->cached) (void)hipFree(kv.second);
+  for (auto& kv : ctx->blocks) (void)hipFree(kv.first);
+  if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+  for (int i = 0; i < VSGM_TIMER_SLOTS; ++i) {
+    if (ctx->ev_start[i]) (void)hipEventDestroy(ctx->ev_start[i]);
+    if (ctx->ev_stop[i]) (void)hipEventDestroy(ctx->ev_stop[i]);
+  }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return VSGM_OK;
+}
+
+int vsgm_ctx_sync(vsgm_ctx* ctx) {
+  if (!ctx) return VSGM_ERR_ARG;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VSGM_OK;
+}
+
+const char* vsgm_last_error(const vsgm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int vsgm_mem_info(const vsgm_ctx* ctx, uint64_t* available, uint64_t* budget, uint64_t* peak) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (available) *available = ctx->budget > ctx->live ? ctx->budget - ctx->live : 0;
+  if (budget) *budget = ctx->budget;
+  if (peak) *peak = ctx->peak;
+  return VSGM_OK;
+}
+
+int vsgm_alloc(vsgm_ctx* ctx, uint64_t bytes, void** out) {
+  if (!ctx || !out) return VSGM_ERR_ARG;
+  return pool_alloc(ctx, bytes, out);
+}
+
+int vsgm_free(vsgm_ctx* ctx, void* ptr) {
+  if (!ctx) return VSGM_ERR_ARG;
+  return pool_free(ctx, ptr);
+}
+
+int vsgm_h2d(vsgm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (bytes == 0) return VSGM_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return VSGM_OK;
+}
+
+int vsgm_d2h(vsgm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (bytes == 0) return VSGM_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VSGM_OK;
+}
+
+int vsgm_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return VSGM_ERR_ARG;
+  if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return VSGM_ERR_OOM;
+  }
+  return VSGM_OK;
+}
+
+int vsgm_host_free(void* ptr) {
+  if (ptr) (void)hipHostFree(ptr);
+  return VSGM_OK;
+}
+
+int vsgm_timer_start(vsgm_ctx* ctx, int slot) {
+  if (!ctx || slot < 0 || slot >= VSGM_TIMER_SLOTS) return VSGM_ERR_ARG;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev_start[slot], ctx->stream));
+  return VSGM_OK;
+}
+
+int vsgm_timer_stop(vsgm_ctx* ctx, int slot, float* ms) {
+  if (!ctx || !ms || slot < 0 || slot >= VSGM_TIMER_SLOTS) return VSGM_ERR_ARG;
+  HIP_TRY(ctx, hipEventRecord(ctx->ev_stop[slot], ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev_stop[slot]));
+  HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev_start[slot], ctx->ev_stop[slot]));
+  return VSGM_OK;
+}
+
+int vsgm_scan_excl_u64(vsgm_ctx* ctx, const uint64_t* d_in, uint64_t* d_out, uint64_t n) {
+  if (!ctx || !d_out || (n && !d_in)) return VSGM_ERR_ARG;
+  return scan_impl(ctx, d_in, d_out, n);
+}
+
+int vsgm_reduce_u64(vsgm_ctx* ctx, const uint64_t* d_in, uint64_t n, uint64_t* d_total) {
+  if (!ctx || !d_total || (n && !d_in)) return VSGM_ERR_ARG;
+  if (n == 0) return VSGM_OK;
+  hipLaunchKernelGGL(k_reduce, dim3(grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_in, n,
+                     (unsigned long long*)d_total);
+  return launch_check(ctx, "k_reduce");
+}
+
+int vsgm_gather_u32(vsgm_ctx* ctx, const uint64_t* d_idx, uint64_t n, const uint32_t* d_src, uint32_t* d_dst) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (n == 0) return VSGM_OK;
+  hipLaunchKernelGGL(k_gather<uint32_t>, dim3(grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_idx, n, d_src, d_dst);
+  return launch_check(ctx, "k_gather");
+}
+
+int vsgm_gather_u64(vsgm_ctx* ctx, const uint64_t* d_idx, uint64_t n, const uint64_t* d_src, uint64_t* d_dst) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (n == 0) return VSGM_OK;
+  hipLaunchKernelGGL(k_gather<uint64_t>, dim3(grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_idx, n, d_src, d_dst);
+  return launch_check(ctx, "k_gather");
+}
+
+int vsgm_iota_u64(vsgm_ctx* ctx, uint64_t* d, uint64_t n, uint64_t base) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (n == 0) return VSGM_OK;
+  hipLaunchKernelGGL(k_iota, dim3(grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d, n, base);
+  return launch_check(ctx, "k_iota");
+}
+
+int vsgm_fill_u64(vsgm_ctx* ctx, uint64_t* d, uint64_t n, uint64_t value) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (n == 0) return VSGM_OK;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d, n, value);
+  return launch_check(ctx, "k_fill");
+}
+
+int vsgm_intersect_write(vsgm_ctx* ctx, const vsgm_level_args* a, vsgm_csr_out* out) {
+  VSGM_TRY(check_args(ctx, a));
+  if (!out) return fail(ctx, VSGM_ERR_ARG, "null output");
+  VSGM_TRY(check_intersect(ctx, a));
+  return run_csr<OP_IWRITE>(ctx, *a, out);
+}
+
+int vsgm_intersect_count_total(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_total) {
+  VSGM_TRY(check_args(ctx, a));
+  if (!d_total) return fail(ctx, VSGM_ERR_ARG, "null total");
+  VSGM_TRY(check_intersect(ctx, a));
+  if (a->other >= 0) VSGM_TRY(check_cand(ctx, a, (uint32_t)a->other));
+  return run_count<OP_ICOUNT>(ctx, *a, d_total, nullptr);
+}
+
+int vsgm_intersect_count_paths(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_counts) {
+  VSGM_TRY(check_args(ctx, a));
+  if (!d_counts && a->path_num) return fail(ctx, VSGM_ERR_ARG, "null counts");
+  VSGM_TRY(check_intersect(ctx, a));
+  VSGM_TRY(vsgm_fill_u64(ctx, d_counts, a->path_num, 0));
+  return run_count<OP_IPATH>(ctx, *a, nullptr, d_counts);
+}
+
+int vsgm_expand(vsgm_ctx* ctx, const vsgm_level_args* a, const uint32_t* mat, uint32_t n_mat, const uint32_t* cu, uint32_t n_cu,
+                vsgm_expand_out* out) {
+  VSGM_TRY(check_args(ctx, a));
+  if (!out || n_mat > VSGM_MAX_QV || n_cu > VSGM_MAX_QV) return fail(ctx, VSGM_ERR_ARG, "bad expand outputs");
+  VSGM_TRY(check_cand(ctx, a, a->target));
+  ExpandCols cols{};
+  cols.n_mat = n_mat;
+  cols.n_cu = n_cu;
+  for (uint32_t i = 0; i < n_mat; ++i) {
+    if (mat[i] >= VSGM_MAX_QV || !a->inst[mat[i]] || mat[i] == a->target) return fail(ctx, VSGM_ERR_ARG, "bad materialized column");
+    cols.mat[i] = mat[i];
+  }
+  for (uint32_t i = 0; i < n_cu; ++i) {
+    if (cu[i] >= VSGM_MAX_QV || cu[i] == a->target) return fail(ctx, VSGM_ERR_ARG, "bad candidate column");
+    cols.cu[i] = cu[i];
+  }
+  memset(out, 0, sizeof(*out));
+  const uint64_t P = a->path_num;
+  ctx->stats = {P, 0, 0};
+  Temps t(ctx);
+  uint64_t *seg_begin = nullptr, *seg_off = nullptr, W = 0;
+  if (P) VSGM_TRY(run_setup_scan<OP_EXPAND>(ctx, *a, t, &seg_begin, &seg_off, &W));
+  ctx->stats.work = W;
+  uint64_t count = 0, *mask = nullptr, *chunk_off = nullptr;
+  if (W) VSGM_TRY(run_mark<OP_EXPAND>(ctx, *a, t, W, seg_begin, seg_off, &mask, &chunk_off, &count));
+  ctx->stats.outputs = count;
+  // output columns
+  std::vector<void*> made;
+  auto undo = [&](int code) {
+    for (void* p : made) pool_free(ctx, p);
+    memset(out, 0, sizeof(*out));
+    return code;
+  };
+  const uint64_t n_alloc = std::max<uint64_t>(count, 1);
+  int s = pool_alloc(ctx, n_alloc * sizeof(uint32_t), (void**)&cols.out_inst[a->target]);
+  if (s != VSGM_OK) return undo(s);
+  made.push_back(cols.out_inst[a->target]);
+  for (uint32_t i = 0; i < n_mat; ++i) {
+    s = pool_alloc(ctx, n_alloc * sizeof(uint32_t), (void**)&cols.out_inst[mat[i]]);
+    if (s != VSGM_OK) return undo(s);
+    made.push_back(cols.out_inst[mat[i]]);
+  }
+  for (uint32_t i = 0; i < n_cu; ++i) {
+    s = pool_alloc(ctx, n_alloc * sizeof(uint64_t), (void**)&cols.out_idx[cu[i]]);
+    if (s != VSGM_OK) return undo(s);
+    made.push_back(cols.out_idx[cu[i]]);
+  }
+  if (count) {
+    s = run_compact<OP_EXPAND>(ctx, *a, nullptr, nullptr, cols, t, W, seg_begin, seg_off, mask, chunk_off);
+    if (s != VSGM_OK) return undo(s);
+  }
+  out->path_num = count;
+  for (int u = 0; u < VSGM_MAX_QV; ++u) {
+    out->inst[u] = cols.out_inst[u];
+    out->cand_idx[u] = cols.out_idx[u];
+  }
+  return VSGM_OK;
+}
+
+int vsgm_filter(vsgm_ctx* ctx, const vsgm_level_args* a, vsgm_csr_out* out) {
+  VSGM_TRY(check_args(ctx, a));
+  if (!out) return fail(ctx, VSGM_ERR_ARG, "null output");
+  VSGM_TRY(check_cand(ctx, a, a->target));
+  return run_csr<OP_FILTER>(ctx, *a, out);
+}
+
+int vsgm_count_sets(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_total) {
+  VSGM_TRY(check_args(ctx, a));
+  if (!d_total) return fail(ctx, VSGM_ERR_ARG, "null total");
+  VSGM_TRY(check_cand(ctx, a, a->target));
+  if (a->other < 0) {
+    ctx->stats = {a->path_num, a->path_num, 0};
+    if (a->path_num == 0) return VSGM_OK;
+    hipLaunchKernelGGL(k_count1, dim3(grid_for((a->path_num + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, *a,
+                       (unsigned long long*)d_total);
+    return launch_check(ctx, "k_count1");
+  }
+  VSGM_TRY(check_cand(ctx, a, (uint32_t)a->other));
+  return run_count<OP_COUNT2>(ctx, *a, d_total, nullptr);
+}
+
+int vsgm_children_count(vsgm_ctx* ctx, const vsgm_level_args* a, int mode, uint64_t* d_out) {
+  VSGM_TRY(check_args(ctx, a));
+  if (mode < 0 || mode > 2 || (!d_out && a->path_num)) return fail(ctx, VSGM_ERR_ARG, "bad children-count mode");
+  if (mode == 0) VSGM_TRY(check_intersect(ctx, a));
+  if (mode == 1) VSGM_TRY(check_cand(ctx, a, a->target));
+  if (a->path_num == 0) return VSGM_OK;
+  hipLaunchKernelGGL(k_children, dim3(grid_for((a->path_num + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, *a, mode, d_out);
+  return launch_check(ctx, "k_children");
+}
+
+int vsgm_split_batch(vsgm_ctx* ctx, const uint64_t* d_children, uint64_t n, uint64_t parent_factor, uint64_t children_factor,
+                     uint64_t batch_bytes, uint64_t* h_ends, uint64_t max_batches, uint64_t* n_batches) {
+  if (!ctx || !n_batches || batch_bytes == 0) return VSGM_ERR_ARG;
+  *n_batches = 0;
+  if (n == 0) return VSGM_OK;
+  Temps t(ctx);
+  uint64_t* prefix;
+  VSGM_TRY(t.get(n + 1, &prefix));
+  hipLaunchKernelGGL(k_cost, dim3(grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_children, n, parent_factor,
+                     children_factor, prefix);
+  VSGM_TRY(launch_check(ctx, "k_cost"));
+  VSGM_TRY(scan_impl(ctx, prefix, prefix, n));
+  uint64_t total;
+  VSGM_TRY(read_u64(ctx, prefix + n, &total));
+  uint64_t nb = (total + batch_bytes - 1) / batch_bytes;
+  std::vector<uint64_t> ends;
+  if (nb > 1) {
+    uint64_t* d_ends;
+    VSGM_TRY(t.get(nb - 1, &d_ends));
+    hipLaunchKernelGGL(k_split, dim3(grid_for((nb - 1 + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, prefix, n, batch_bytes, nb - 1,
+                       d_ends);
+    VSGM_TRY(launch_check(ctx, "k_split"));
+    std::vector<uint64_t> raw(nb - 1);
+    VSGM_TRY(vsgm_d2h(ctx, raw.data(), d_ends, (nb - 1) * sizeof(uint64_t)));
+    uint64_t prev = 0;
+    for (uint64_t e : raw) {
+      if (e <= prev) e = prev + 1;  // every batch keeps at least one parent
+      if (e >= n) break;
+      ends.push_back(e);
+      prev = e;
+    }
+  }
+  ends.push_back(n);
+  *n_batches = ends.size();
+  if (!h_ends || ends.size() > max_batches) return fail(ctx, VSGM_ERR_ARG, "split needs %llu batch slots", (unsigned long long)ends.size());
+  std::copy(ends.begin(), ends.end(), h_ends);
+  return VSGM_OK;
+}
+
+int vsgm_last_op_stats(const vsgm_ctx* ctx, vsgm_op_stats* out) {
+  if (!ctx || !out) return VSGM_ERR_ARG;
+  *out = ctx->stats;
+  return VSGM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,372 @@
+// C-ABI of the host library (include/vsgm.h) over the C++ classes.
+#include "vsgm.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "executor.h"
+#include "graph.h"
+#include "plan.h"
+#include "query.h"
+#include "view.h"
+
+struct vsgm_graph {
+  vsgm::Graph g;
+};
+struct vsgm_query {
+  vsgm::Query q;
+};
+struct vsgm_plan {
+  std::unique_ptr<vsgm::Plan> p;
+};
+struct vsgm_view_manager {
+  std::unique_ptr<vsgm::ViewBinManager> m;
+};
+struct vsgm_view_holder {
+  std::unique_ptr<vsgm::ViewBinHolder> h;
+};
+struct vsgm_executor {
+  std::unique_ptr<vsgm::PipelineExecutor> e;
+};
+
+namespace {
+thread_local std::string g_error;
+
+template <typename F>
+int Guard(F&& f) {
+  try {
+    g_error.clear();
+    f();
+    return VSGM_OK;
+  } catch (const std::bad_alloc&) {
+    g_error = "host allocation failed";
+    return VSGM_ERR_OOM;
+  } catch (const std::exception& ex) {
+    g_error = ex.what();
+    return VSGM_ERR_ARG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* vsgm_error(void) { return g_error.c_str(); }
+
+int vsgm_graph_read_bin(const char* path, vsgm_graph** out) {
+  return Guard([&] { *out = new vsgm_graph{vsgm::ReadCSRBinFile(path)}; });
+}
+
+int vsgm_graph_write_bin(const vsgm_graph* g, const char* path) {
+  return Guard([&] { vsgm::WriteCSRBinFile(path, g->g); });
+}
+
+int vsgm_graph_from_csr(uint64_t vertex_count, const uint64_t* row_ptrs, const uint32_t* cols, vsgm_graph** out) {
+  return Guard([&] {
+    auto* h = new vsgm_graph;
+    h->g.vertex_count = vertex_count;
+    h->g.row_ptrs.assign(row_ptrs, row_ptrs + vertex_count + 1);
+    h->g.cols.assign(cols, cols + row_ptrs[vertex_count]);
+    *out = h;
+  });
+}
+
+int vsgm_graph_from_edges(uint64_t vertex_count, const uint32_t* pairs, uint64_t n_pairs, vsgm_graph** out) {
+  return Guard([&] {
+    std::vector<std::pair<vsgm::uintV, vsgm::uintV>> edges(n_pairs);
+    for (uint64_t i = 0; i < n_pairs; ++i) {
+      VSGM_CHECK(pairs[2 * i] < vertex_count && pairs[2 * i + 1] < vertex_count, "edge endpoint out of range");
+      edges[i] = {pairs[2 * i], pairs[2 * i + 1]};
+    }
+    *out = new vsgm_graph{vsgm::BuildFromEdges(vertex_count, edges)};
+  });
+}
+
+int vsgm_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads, vsgm_graph** out) {
+  return Guard([&] { *out = new vsgm_graph{vsgm::GenerateRMAT(scale, edge_factor, a, b, c, seed, threads)}; });
+}
+
+int vsgm_graph_chung_lu(uint64_t vertex_count, uint64_t draws, double gamma, uint64_t seed, int threads, vsgm_graph** out) {
+  return Guard([&] { *out = new vsgm_graph{vsgm::GenerateChungLu(vertex_count, draws, gamma, seed, threads)}; });
+}
+
+int vsgm_graph_info(const vsgm_graph* g, uint64_t* vertex_count, uint64_t* entries, uint64_t* max_degree) {
+  return Guard([&] {
+    if (vertex_count) *vertex_count = g->g.vertex_count;
+    if (entries) *entries = g->g.cols.size();
+    if (max_degree) *max_degree = g->g.MaxDegree();
+  });
+}
+
+int vsgm_graph_csr(const vsgm_graph* g, const uint64_t** row_ptrs, const uint32_t** cols) {
+  return Guard([&] {
+    *row_ptrs = g->g.row_ptrs.data();
+    *cols = g->g.cols.data();
+  });
+}
+
+int vsgm_graph_validate(const vsgm_graph* g, char* msg, size_t len) {
+  return Guard([&] {
+    std::string m = g->g.Validate(true);
+    if (len) {
+      strncpy(msg, m.c_str(), len - 1);
+      msg[len - 1] = 0;
+    }
+  });
+}
+
+void vsgm_graph_free(vsgm_graph* g) { delete g; }
+
+int vsgm_query_preset(int preset, vsgm_query** out) {
+  return Guard([&] { *out = new vsgm_query{vsgm::Query::Preset(preset)}; });
+}
+
+int vsgm_query_clique(int k, vsgm_query** out) {
+  return Guard([&] { *out = new vsgm_query{vsgm::Query::Clique(k)}; });
+}
+
+int vsgm_query_from_matrix(int n, const uint8_t* adj, vsgm_query** out) {
+  return Guard([&] {
+    std::vector<std::vector<int>> lists(n);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        if (i != j && adj[i * n + j]) {
+          VSGM_CHECK(adj[j * n + i], "adjacency matrix must be symmetric");
+          lists[i].push_back(j);
+        }
+    *out = new vsgm_query{vsgm::Query::FromAdjacency(lists)};
+  });
+}
+
+int vsgm_query_graphlets(int k, vsgm_query** out, int cap, int* n_out) {
+  return Guard([&] {
+    auto qs = vsgm::Query::Graphlets(k);
+    *n_out = (int)qs.size();
+    for (int i = 0; i < (int)qs.size() && i < cap; ++i) out[i] = new vsgm_query{qs[i]};
+  });
+}
+
+int vsgm_query_info(const vsgm_query* q, int* n, int* edges, uint8_t* adj, uint8_t* cond) {
+  return Guard([&] {
+    int k = (int)q->q.GetVertexCount();
+    if (n) *n = k;
+    if (edges) *edges = (int)q->q.GetEdgeCount();
+    if (adj) {
+      memset(adj, 0, k * k);
+      for (int i = 0; i < k; ++i)
+        for (auto j : q->q.GetConnectivity()[i]) adj[i * k + j] = 1;
+    }
+    if (cond)
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) cond[i * k + j] = (uint8_t)vsgm::GetConditionType(i, j, q->q.GetOrder());
+  });
+}
+
+void vsgm_query_free(vsgm_query* q) { delete q; }
+
+int vsgm_plan_optimize(const vsgm_query* q, vsgm_plan** out) {
+  return Guard([&] {
+    auto p = std::make_unique<vsgm::Plan>(q->q);
+    p->Optimize();
+    *out = new vsgm_plan{std::move(p)};
+  });
+}
+
+int vsgm_plan_from_order(const vsgm_query* q, const uint32_t* seq, int n, vsgm_plan** out) {
+  return Guard([&] {
+    std::vector<vsgm::uintV> s(seq, seq + n);
+    *out = new vsgm_plan{std::make_unique<vsgm::Plan>(vsgm::Plan::FromOrder(q->q, s))};
+  });
+}
+
+int vsgm_plan_info(const vsgm_plan* p, int* hop, uint64_t* root_degree, int* n_ops) {
+  return Guard([&] {
+    if (hop) *hop = p->p->GetHop();
+    if (root_degree) *root_degree = p->p->GetRootDegree();
+    if (n_ops) *n_ops = (int)p->p->GetExecuteOperations().size();
+  });
+}
+
+int vsgm_plan_op(const vsgm_plan* p, int i, int* op, int* vertex) {
+  return Guard([&] {
+    const auto& ops = p->p->GetExecuteOperations();
+    VSGM_CHECK(i >= 0 && i < (int)ops.size(), "operation index out of range");
+    *op = (int)ops[i].first;
+    *vertex = (int)ops[i].second;
+  });
+}
+
+int vsgm_plan_order(const vsgm_plan* p, uint32_t* seq, int cap) {
+  return Guard([&] {
+    const auto& s = p->p->GetSearchSequence();
+    VSGM_CHECK(cap >= (int)s.size(), "order buffer too small");
+    for (size_t i = 0; i < s.size(); ++i) seq[i] = s[i];
+  });
+}
+
+int vsgm_plan_serialize(const vsgm_plan* p, char* buf, size_t len, size_t* needed) {
+  return Guard([&] {
+    std::string s = p->p->Serialize();
+    if (needed) *needed = s.size() + 1;
+    if (buf && len) {
+      size_t n = std::min(len - 1, s.size());
+      memcpy(buf, s.data(), n);
+      buf[n] = 0;
+    }
+  });
+}
+
+void vsgm_plan_free(vsgm_plan* p) { delete p; }
+
+int vsgm_vbmap_write(const char* path, int n_bins, const int* assignment, uint64_t vertex_count) {
+  return Guard([&] { vsgm::WriteViewBinPartition(path, n_bins, std::vector<int>(assignment, assignment + vertex_count)); });
+}
+
+int vsgm_view_manager_load(const vsgm_graph* g, int hop, int threads, const char* vbmap, vsgm_view_manager** out) {
+  return Guard([&] {
+    auto m = std::make_unique<vsgm::ViewBinManager>(&g->g, hop, threads);
+    m->LoadViewBinPartition(vbmap);
+    *out = new vsgm_view_manager{std::move(m)};
+  });
+}
+
+int vsgm_view_manager_from_assignment(const vsgm_graph* g, int hop, int threads, int n_bins, const int* assignment,
+                                      vsgm_view_manager** out) {
+  return Guard([&] {
+    auto m = std::make_unique<vsgm::ViewBinManager>(&g->g, hop, threads);
+    m->SetPartition(n_bins, std::vector<int>(assignment, assignment + g->g.vertex_count));
+    *out = new vsgm_view_manager{std::move(m)};
+  });
+}
+
+int vsgm_view_manager_info(const vsgm_view_manager* m, int* n_bins, uint64_t* max_sources) {
+  return Guard([&] {
+    if (n_bins) *n_bins = m->m->GetViewBinNumber();
+    if (max_sources) *max_sources = m->m->GetMaxPartitionedSourcesNum();
+  });
+}
+
+int vsgm_view_manager_reorder(vsgm_view_manager* m) {
+  return Guard([&] { m->m->Reorder(); });
+}
+
+int vsgm_view_manager_split(vsgm_view_manager* m, int devices, int times) {
+  return Guard([&] { m->m->Split(devices, times); });
+}
+
+void vsgm_view_manager_free(vsgm_view_manager* m) { delete m; }
+
+int vsgm_view_holder_create(uint64_t vertex_count, uint64_t max_bytes, vsgm_view_holder** out) {
+  return Guard([&] { *out = new vsgm_view_holder{std::make_unique<vsgm::ViewBinHolder>(vertex_count, max_bytes)}; });
+}
+
+int vsgm_view_materialize(const vsgm_view_manager* m, int index, vsgm_view_holder* h, uint64_t root_degree) {
+  return Guard([&] {
+    const auto& pool = m->m->GetViewBinPool();
+    VSGM_CHECK(index >= 0 && index < (int)pool.size(), "view bin index out of range");
+    pool[index]->Materialize(*h->h, root_degree);
+  });
+}
+
+int vsgm_view_holder_info(const vsgm_view_holder* h, int* bin_id, uint64_t* sources, uint64_t* entries, uint64_t* vertices) {
+  return Guard([&] {
+    if (bin_id) *bin_id = h->h->GetId();
+    if (sources) *sources = h->h->GetSourcesNum();
+    if (entries) *entries = h->h->GetTotalSize();
+    if (vertices) *vertices = h->h->GetVerticesNum();
+  });
+}
+
+int vsgm_view_holder_csr(const vsgm_view_holder* h, const uint64_t** row_ptrs, const uint32_t** cols, const uint32_t** sources) {
+  return Guard([&] {
+    if (row_ptrs) *row_ptrs = h->h->GetRowPtrs();
+    if (cols) *cols = h->h->GetCols();
+    if (sources) *sources = h->h->GetSources().data();
+  });
+}
+
+void vsgm_view_holder_free(vsgm_view_holder* h) { delete h; }
+
+int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* const* plans, int n_plans, uint64_t max_sources,
+                         uint64_t mem_budget, vsgm_executor** out) {
+  int code = VSGM_OK;
+  int s = Guard([&] {
+    std::vector<const vsgm::Plan*> ps;
+    for (int i = 0; i < n_plans; ++i) ps.push_back(plans[i]->p.get());
+    vsgm::ExecutorOptions opt;
+    opt.mem_budget = mem_budget;
+    int n_dev = 0;
+    vsgm_device_count(&n_dev);
+    if (n_dev == 0) {
+      code = VSGM_ERR_NO_DEVICE;
+      throw vsgm::Error("no HIP device is visible: the enumerator needs a GPU");
+    }
+    *out = new vsgm_executor{std::make_unique<vsgm::PipelineExecutor>(device, vertex_count, ps, max_sources, opt)};
+  });
+  return code != VSGM_OK ? code : s;
+}
+
+int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
+  return Guard([&] {
+    auto& o = e->e->Options();
+    std::string k = key;
+    if (k == "batch_bytes") o.batch_bytes = (uint64_t)value;
+    else if (k == "min_batch_bytes") o.min_batch_bytes = (uint64_t)value;
+    else if (k == "profile") o.profile = value != 0;
+    else throw vsgm::Error("unknown executor option " + k);
+  });
+}
+
+int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h) {
+  return Guard([&] { e->e->Transfer(*h->h); });
+}
+
+int vsgm_executor_transfer_graph(vsgm_executor* e, const vsgm_graph* g, const uint32_t* sources, uint64_t n_sources) {
+  return Guard([&] {
+    std::vector<vsgm::uintV> s;
+    if (sources) s.assign(sources, sources + n_sources);
+    e->e->TransferGraph(g->g, s);
+  });
+}
+
+int vsgm_executor_match(vsgm_executor* e, int do_match) {
+  return Guard([&] { e->e->Match(do_match != 0); });
+}
+
+int vsgm_executor_counts(const vsgm_executor* e, uint64_t* counts, int n) {
+  return Guard([&] {
+    const auto& c = e->e->GetTotalCounts();
+    VSGM_CHECK(n >= (int)c.size(), "counts buffer too small");
+    for (size_t i = 0; i < c.size(); ++i) counts[i] = c[i];
+  });
+}
+
+int vsgm_executor_match_ms(const vsgm_executor* e, double* ms, int n) {
+  return Guard([&] {
+    const auto& c = e->e->GetMatchMs();
+    VSGM_CHECK(n >= (int)c.size(), "buffer too small");
+    for (size_t i = 0; i < c.size(); ++i) ms[i] = c[i];
+  });
+}
+
+int vsgm_executor_reset(vsgm_executor* e) {
+  return Guard([&] { e->e->ResetCounts(); });
+}
+
+int vsgm_executor_level_stats(const vsgm_executor* e, int plan, vsgm_level_stat* out, int cap, int* n_out) {
+  return Guard([&] {
+    const auto& all = e->e->GetLevelStats();
+    VSGM_CHECK(plan >= 0 && plan < (int)all.size(), "plan index out of range");
+    const auto& s = all[plan];
+    *n_out = (int)s.size();
+    for (int i = 0; i < (int)s.size() && i < cap; ++i) out[i] = vsgm_level_stat{s[i].batches, s[i].paths, s[i].work, s[i].outputs, s[i].ms};
+  });
+}
+
+int vsgm_executor_peak_bytes(const vsgm_executor* e, uint64_t* bytes) {
+  return Guard([&] { *bytes = e->e->PeakDeviceBytes(); });
+}
+
+void vsgm_executor_free(vsgm_executor* e) { delete e; }
+
+}  // extern "C"

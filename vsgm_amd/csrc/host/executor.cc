@@ -1,0 +1,410 @@
+// Pipeline executor implementation. See executor.h.
+#include "executor.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace vsgm {
+
+// Pool allocation owned by a shared_ptr; level columns share it.
+class DeviceBuffer {
+ public:
+  DeviceBuffer(vsgm_ctx* ctx, void* adopt) : ctx_(ctx), ptr_(adopt) {}
+  DeviceBuffer(vsgm_ctx* ctx, uint64_t bytes) : ctx_(ctx) {
+    int s = vsgm_alloc(ctx, bytes, &ptr_);
+    if (s != VSGM_OK) throw Error(std::string("device allocation failed: ") + vsgm_last_error(ctx));
+  }
+  ~DeviceBuffer() {
+    if (ptr_) vsgm_free(ctx_, ptr_);
+  }
+  void* get() const { return ptr_; }
+
+ private:
+  vsgm_ctx* ctx_;
+  void* ptr_ = nullptr;
+};
+
+namespace {
+template <typename T>
+struct Column {
+  std::shared_ptr<DeviceBuffer> owner;
+  T* ptr = nullptr;
+  uint64_t size = 0;
+};
+
+template <typename T>
+Column<T> Adopt(vsgm_ctx* ctx, T* p, uint64_t size) {
+  Column<T> c;
+  c.owner = std::make_shared<DeviceBuffer>(ctx, (void*)p);
+  c.ptr = p;
+  c.size = size;
+  return c;
+}
+
+bool Contains(const VTGroup& g, uintV u) { return std::find(g.begin(), g.end(), u) != g.end(); }
+}  // namespace
+
+struct PipelineExecutor::Level {
+  uint64_t path_num = 0;
+  Column<uint32_t> inst[kMaxQueryVerticesNum];
+  Column<uint32_t> cand[kMaxQueryVerticesNum];
+  Column<uint64_t> cand_off[kMaxQueryVerticesNum];
+  Column<uint64_t> cand_idx[kMaxQueryVerticesNum];  // ptr == nullptr: identity rows
+  uint64_t cand_idx_base[kMaxQueryVerticesNum] = {0};
+
+  // Non-owning batch slice [l, l+count) of the path-indexed columns
+  // (ImData::CopyBatchData, im_data.h:22-45); candidate CSRs stay whole.
+  Level Slice(uint64_t l, uint64_t count, const VTGroup& mat, const VTGroup& cu) const {
+    Level s;
+    s.path_num = count;
+    for (auto u : mat) {
+      s.inst[u] = inst[u];
+      s.inst[u].ptr += l;
+      s.inst[u].size = count;
+    }
+    for (auto v : cu) {
+      s.cand[v] = cand[v];
+      s.cand_off[v] = cand_off[v];
+      s.cand_idx[v] = cand_idx[v];
+      s.cand_idx_base[v] = cand_idx_base[v];
+      if (cand_idx[v].ptr) {
+        s.cand_idx[v].ptr += l;
+        s.cand_idx[v].size = count;
+      } else {
+        s.cand_idx_base[v] += l;
+      }
+    }
+    return s;
+  }
+};
+
+PipelineExecutor::PipelineExecutor(int device, uint64_t vertex_count, std::vector<const Plan*> plans, uint64_t max_sources,
+                                   ExecutorOptions opt)
+    : device_(device), vertex_count_(vertex_count), opt_(opt) {
+  (void)max_sources;
+  int s = vsgm_ctx_create(device, opt_.mem_budget, &ctx_);
+  if (s == VSGM_ERR_NO_DEVICE) throw Error("no HIP device is visible: the enumerator needs a GPU");
+  if (s != VSGM_OK) throw Error("cannot create the device context on device " + std::to_string(device) + " (status " + std::to_string(s) + ")");
+  d_row_ptrs_ = std::make_shared<DeviceBuffer>(ctx_, (vertex_count_ + 1) * sizeof(uintE));
+  SetPlans(std::move(plans));
+}
+
+PipelineExecutor::~PipelineExecutor() {
+  d_row_ptrs_.reset();
+  d_cols_.reset();
+  d_sources_.reset();
+  if (d_count_) vsgm_free(ctx_, d_count_);
+  vsgm_ctx_destroy(ctx_);
+}
+
+void PipelineExecutor::SetPlans(std::vector<const Plan*> plans) {
+  plans_ = std::move(plans);
+  for (auto* p : plans_) VSGM_CHECK(p && p->GetVertexCount() <= (size_t)kMaxQueryVerticesNum, "bad plan");
+  ResetCounts();
+}
+
+void PipelineExecutor::ResetCounts() {
+  total_counts_.assign(plans_.size(), 0);
+  match_ms_.assign(plans_.size(), 0.0);
+  level_stats_.assign(plans_.size(), {});
+  for (size_t i = 0; i < plans_.size(); ++i) level_stats_[i].assign(plans_[i]->GetExecuteOperations().size(), LevelStat{});
+}
+
+uint64_t PipelineExecutor::PeakDeviceBytes() const {
+  uint64_t peak = 0;
+  vsgm_mem_info(ctx_, nullptr, nullptr, &peak);
+  return peak;
+}
+
+void PipelineExecutor::Check(int status, const char* what) const {
+  if (status != VSGM_OK) throw Error(std::string(what) + " failed: " + vsgm_last_error(ctx_));
+}
+
+void PipelineExecutor::Transfer(const ViewBinHolder& vbh) {
+  VSGM_CHECK(vbh.GetVertexCount() == vertex_count_, "view vertex count does not match the executor");
+  Check(vsgm_h2d(ctx_, d_row_ptrs_->get(), vbh.GetRowPtrs(), (vertex_count_ + 1) * sizeof(uintE)), "row_ptrs upload");
+  if (!d_cols_ || cols_size_ < vbh.GetTotalSize()) {
+    d_cols_.reset();
+    d_cols_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(vbh.GetTotalSize(), 1) * sizeof(uintV));
+    cols_size_ = vbh.GetTotalSize();
+  }
+  Check(vsgm_h2d(ctx_, d_cols_->get(), vbh.GetCols(), vbh.GetTotalSize() * sizeof(uintV)), "cols upload");
+  sources_num_ = vbh.GetSourcesNum();
+  d_sources_.reset();
+  d_sources_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources_num_, 1) * sizeof(uintV));
+  Check(vsgm_h2d(ctx_, d_sources_->get(), vbh.GetSources().data(), sources_num_ * sizeof(uintV)), "sources upload");
+  Check(vsgm_ctx_sync(ctx_), "view upload");
+}
+
+void PipelineExecutor::TransferGraph(const Graph& g, const std::vector<uintV>& sources) {
+  VSGM_CHECK(g.vertex_count == vertex_count_, "graph vertex count does not match the executor");
+  Check(vsgm_h2d(ctx_, d_row_ptrs_->get(), g.row_ptrs.data(), (vertex_count_ + 1) * sizeof(uintE)), "row_ptrs upload");
+  d_cols_.reset();
+  d_cols_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(g.cols.size(), 1) * sizeof(uintV));
+  cols_size_ = g.cols.size();
+  Check(vsgm_h2d(ctx_, d_cols_->get(), g.cols.data(), g.cols.size() * sizeof(uintV)), "cols upload");
+  std::vector<uintV> all;
+  const std::vector<uintV>* src = &sources;
+  if (sources.empty()) {
+    all.resize(vertex_count_);
+    for (uint64_t v = 0; v < vertex_count_; ++v) all[v] = (uintV)v;
+    src = &all;
+  }
+  sources_num_ = src->size();
+  d_sources_.reset();
+  d_sources_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources_num_, 1) * sizeof(uintV));
+  Check(vsgm_h2d(ctx_, d_sources_->get(), src->data(), sources_num_ * sizeof(uintV)), "sources upload");
+  Check(vsgm_ctx_sync(ctx_), "graph upload");
+}
+
+void PipelineExecutor::Match(bool do_match) {
+  if (!do_match) return;
+  VSGM_CHECK(d_cols_ && d_sources_, "Match before Transfer");
+  if (!d_count_) Check(vsgm_alloc(ctx_, sizeof(uint64_t), (void**)&d_count_), "count allocation");
+  for (size_t i = 0; i < plans_.size(); ++i) {
+    plan_ = plans_[i];
+    plan_index_ = i;
+    Check(vsgm_fill_u64(ctx_, d_count_, 1, 0), "count reset");
+    Check(vsgm_timer_start(ctx_, 0), "timer");
+    ProcessLevel(0, Level{});
+    float ms = 0;
+    Check(vsgm_timer_stop(ctx_, 0, &ms), "timer");
+    uint64_t c = 0;
+    Check(vsgm_d2h(ctx_, &c, d_count_, sizeof(uint64_t)), "count readback");
+    total_counts_[i] += c;
+    match_ms_[i] += ms;
+  }
+  plan_ = nullptr;
+}
+
+uint64_t PipelineExecutor::LevelBatchBytes(size_t level) const {
+  if (opt_.batch_bytes) return opt_.batch_bytes;
+  uint64_t available = 0;
+  vsgm_mem_info(ctx_, &available, nullptr, nullptr);
+  size_t remaining = plan_->GetExecuteOperations().size() - level;
+  uint64_t b = (uint64_t)(0.9 * (double)(available / std::max<size_t>(remaining, 1)));
+  return std::max(b, opt_.min_batch_bytes);
+}
+
+void PipelineExecutor::FillArgs(size_t level, const Level& l, vsgm_level_args& a) const {
+  memset(&a, 0, sizeof(a));
+  a.row_ptrs = (const uint64_t*)d_row_ptrs_->get();
+  a.cols = (const uint32_t*)d_cols_->get();
+  a.vertex_count = vertex_count_;
+  a.path_num = l.path_num;
+  for (int u = 0; u < kMaxQueryVerticesNum; ++u) {
+    a.inst[u] = l.inst[u].ptr;
+    a.cand[u] = l.cand[u].ptr;
+    a.cand_off[u] = l.cand_off[u].ptr;
+    a.cand_idx[u] = l.cand_idx[u].ptr;
+    a.cand_idx_base[u] = l.cand_idx_base[u];
+  }
+  const auto& e = plan_->GetExecuteOperations()[level];
+  const auto& cu = plan_->GetComputedUnmaterializedVertices()[level];
+  a.target = e.second;
+  a.other = -1;
+  auto set_conds = [&a](const std::vector<CondType>& c) {
+    a.n_cond = (uint32_t)c.size();
+    for (size_t i = 0; i < c.size(); ++i) a.cond[i] = vsgm_cond{(uint32_t)c[i].first, c[i].second};
+  };
+  switch (e.first) {
+    case COMPUTE:
+    case COMPUTE_COUNT:
+    case COMPUTE_PATH_COUNT: {
+      const auto& back = plan_->GetBackwardConnectivity()[e.second];
+      a.n_back = (uint32_t)back.size();
+      for (size_t i = 0; i < back.size(); ++i) a.back[i] = back[i];
+      set_conds(plan_->GetComputeConditions()[e.second]);
+      if (e.first == COMPUTE_COUNT && cu.size() == 1) {
+        a.other = (int32_t)cu[0];
+        a.other_op = (uint32_t)plan_->ConditionBetween(e.second, cu[0]);
+      }
+      VSGM_CHECK(e.first != COMPUTE_COUNT || cu.size() <= 1, "COMPUTE_COUNT with more than one pending candidate set");
+    } break;
+    case MATERIALIZE:
+      set_conds(plan_->GetMaterializeConditions()[e.second]);
+      break;
+    case FILTER_COMPUTE:
+      set_conds(plan_->GetFilterConditions()[e.second]);
+      break;
+    case COUNT:
+      VSGM_CHECK(cu.size() == 1 || cu.size() == 2, "COUNT needs one or two candidate sets");
+      a.target = cu[0];
+      if (cu.size() == 2) {
+        a.other = (int32_t)cu[1];
+        a.other_op = (uint32_t)plan_->ConditionBetween(cu[0], cu[1]);
+      }
+      break;
+  }
+}
+
+void PipelineExecutor::ProcessLevel(size_t level, const Level& parent) {
+  const auto& ops = plan_->GetExecuteOperations();
+  if (level == ops.size()) return;
+  auto& stat = level_stats_[plan_index_][level];
+  const size_t remaining = ops.size() - level;
+
+  if (level == 0) {
+    // InitFirstLevel (materialize.h:11-18): root bound to the bin's sources,
+    // split evenly by the level-0 cost (processor.h:207-213).
+    const uint64_t P = sources_num_;
+    if (P == 0) return;
+    const uint64_t pf = sizeof(uintV) + (uint64_t)std::ceil(3.0 * sizeof(uint64_t) / remaining);
+    const uint64_t bytes = LevelBatchBytes(0);
+    uint64_t nb = std::max<uint64_t>(1, (P * pf + bytes - 1) / bytes);
+    uint64_t per = (P + nb - 1) / nb;
+    const uintV root = ops[0].second;
+    for (uint64_t l = 0; l < P; l += per) {
+      Level child;
+      child.path_num = std::min(per, P - l);
+      child.inst[root].owner = d_sources_;
+      child.inst[root].ptr = (uint32_t*)d_sources_->get() + l;
+      child.inst[root].size = child.path_num;
+      stat.batches++;
+      stat.paths += child.path_num;
+      stat.outputs += child.path_num;
+      ProcessLevel(1, child);
+    }
+    return;
+  }
+
+  const uint64_t P = parent.path_num;
+  if (P == 0) return;
+  const auto op = ops[level].first;
+  const auto& mat = plan_->GetMaterializedVertices()[level];
+  const auto& cu = plan_->GetComputedUnmaterializedVertices()[level];
+  const uint64_t bytes = LevelBatchBytes(level);
+
+  // cost model of this implementation's buffers (bytes per parent / child)
+  int mode = 2;
+  uint64_t pf = 16, cf = 0;
+  switch (op) {
+    case COMPUTE:
+    case COMPUTE_PATH_COUNT:
+      mode = 0;
+      pf = 40;
+      cf = 5;
+      break;
+    case MATERIALIZE:
+      mode = 1;
+      pf = 24;
+      cf = 4 * (mat.size() + 1) + 8 * (cu.size() ? cu.size() - 1 : 0) + 17;
+      break;
+    case FILTER_COMPUTE:
+      mode = 1;
+      pf = 32;
+      cf = 5;
+      break;
+    case COMPUTE_COUNT:
+      pf = 16;
+      break;
+    case COUNT:
+      pf = cu.size() == 2 ? 16 : 1;
+      break;
+  }
+
+  std::vector<uint64_t> ends;
+  if (mode == 2) {
+    uint64_t nb = std::max<uint64_t>(1, (P * pf + bytes - 1) / bytes);
+    uint64_t per = (P + nb - 1) / nb;
+    for (uint64_t e = per; e < P; e += per) ends.push_back(e);
+    ends.push_back(P);
+  } else {
+    vsgm_level_args a;
+    FillArgs(level, parent, a);
+    DeviceBuffer children(ctx_, P * sizeof(uint64_t));
+    Check(vsgm_children_count(ctx_, &a, mode, (uint64_t*)children.get()), "children count");
+    ends.resize(64);
+    uint64_t nb = 0;
+    int s = vsgm_split_batch(ctx_, (const uint64_t*)children.get(), P, pf, cf, bytes, ends.data(), ends.size(), &nb);
+    if (s == VSGM_ERR_ARG && nb > ends.size()) {
+      ends.resize(nb);
+      s = vsgm_split_batch(ctx_, (const uint64_t*)children.get(), P, pf, cf, bytes, ends.data(), ends.size(), &nb);
+    }
+    Check(s, "batch split");
+    ends.resize(nb);
+  }
+
+  uint64_t begin = 0;
+  for (uint64_t end : ends) {
+    Level slice = parent.Slice(begin, end - begin, mat, cu);
+    RunBatch(level, slice);
+    begin = end;
+  }
+}
+
+void PipelineExecutor::RunBatch(size_t level, const Level& slice) {
+  const auto& e = plan_->GetExecuteOperations()[level];
+  const auto& mat = plan_->GetMaterializedVertices()[level];
+  const auto& cu = plan_->GetComputedUnmaterializedVertices()[level];
+  auto& stat = level_stats_[plan_index_][level];
+  const uintV u = e.second;
+  vsgm_level_args a;
+  FillArgs(level, slice, a);
+  if (opt_.profile) Check(vsgm_timer_start(ctx_, 1), "timer");
+
+  Level child;
+  bool next = false;
+  switch (e.first) {
+    case COMPUTE: {
+      vsgm_csr_out out;
+      Check(vsgm_intersect_write(ctx_, &a, &out), "intersect");
+      child = slice;
+      child.cand[u] = Adopt(ctx_, out.values, out.count);
+      child.cand_off[u] = Adopt(ctx_, out.offsets, slice.path_num + 1);
+      child.cand_idx[u] = Column<uint64_t>{};
+      child.cand_idx_base[u] = 0;
+      next = out.count > 0;
+    } break;
+    case MATERIALIZE: {
+      std::vector<uint32_t> carried;
+      for (auto v : cu)
+        if (v != u) carried.push_back(v);
+      vsgm_expand_out out;
+      Check(vsgm_expand(ctx_, &a, mat.data(), (uint32_t)mat.size(), carried.data(), (uint32_t)carried.size(), &out), "expand");
+      child.path_num = out.path_num;
+      child.inst[u] = Adopt(ctx_, out.inst[u], out.path_num);
+      for (auto w : mat) child.inst[w] = Adopt(ctx_, out.inst[w], out.path_num);
+      for (auto v : carried) {
+        child.cand[v] = slice.cand[v];
+        child.cand_off[v] = slice.cand_off[v];
+        child.cand_idx[v] = Adopt(ctx_, out.cand_idx[v], out.path_num);
+        child.cand_idx_base[v] = 0;
+      }
+      next = out.path_num > 0;
+    } break;
+    case FILTER_COMPUTE: {
+      vsgm_csr_out out;
+      Check(vsgm_filter(ctx_, &a, &out), "filter");
+      child = slice;
+      child.cand[u] = Adopt(ctx_, out.values, out.count);
+      child.cand_off[u] = Adopt(ctx_, out.offsets, slice.path_num + 1);
+      child.cand_idx[u] = Column<uint64_t>{};
+      child.cand_idx_base[u] = 0;
+      next = out.count > 0;
+    } break;
+    case COMPUTE_COUNT:
+      Check(vsgm_intersect_count_total(ctx_, &a, d_count_), "intersect count");
+      break;
+    case COUNT:
+      Check(vsgm_count_sets(ctx_, &a, d_count_), "count");
+      break;
+    case COMPUTE_PATH_COUNT:
+      throw Error("COMPUTE_PATH_COUNT is not emitted by the planner");
+  }
+  vsgm_op_stats st;
+  vsgm_last_op_stats(ctx_, &st);
+  stat.batches++;
+  stat.paths += slice.path_num;
+  stat.work += st.work;
+  stat.outputs += st.outputs;
+  if (opt_.profile) {
+    float ms = 0;
+    Check(vsgm_timer_stop(ctx_, 1, &ms), "timer");
+    stat.ms += ms;
+  }
+  if (next) ProcessLevel(level + 1, child);
+}
+
+}  // namespace vsgm

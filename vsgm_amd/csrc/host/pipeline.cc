@@ -1,0 +1,224 @@
+// The pm_pipeline driver (src/app/pattern_matching_pipeline.cu:27-201):
+// producers materialise view bins into a ring of pinned holders, one
+// consumer per GPU transfers each bin and matches every plan, and the
+// per-consumer counts are summed at the end. Same flags and output lines.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <memory>
+#include <mutex>
+#include <queue>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "executor.h"
+#include "graph.h"
+#include "plan.h"
+#include "query.h"
+#include "view.h"
+#include "vsgm.h"
+
+namespace vsgm {
+namespace {
+
+// Bounded blocking queue of holder ids (the role of rigtorp::MPMCQueue in
+// pattern_matching_pipeline.cu:106-110).
+class BoundedQueue {
+ public:
+  explicit BoundedQueue(size_t cap) : cap_(cap) {}
+  void push(int v) {
+    std::unique_lock<std::mutex> lk(mu_);
+    not_full_.wait(lk, [&] { return q_.size() < cap_; });
+    q_.push(v);
+    not_empty_.notify_one();
+  }
+  // Blocks until an item arrives or `closed` is set and the queue drained.
+  bool pop(int& v) {
+    std::unique_lock<std::mutex> lk(mu_);
+    not_empty_.wait(lk, [&] { return !q_.empty() || closed_; });
+    if (q_.empty()) return false;
+    v = q_.front();
+    q_.pop();
+    not_full_.notify_one();
+    return true;
+  }
+  void close() {
+    std::lock_guard<std::mutex> lk(mu_);
+    closed_ = true;
+    not_empty_.notify_all();
+  }
+
+ private:
+  size_t cap_;
+  bool closed_ = false;
+  std::queue<int> q_;
+  std::mutex mu_;
+  std::condition_variable not_full_, not_empty_;
+};
+
+struct Flags {
+  int argc;
+  char** argv;
+  const char* raw(const std::string& k) const {
+    for (int i = 1; i + 1 < argc; ++i)
+      if (k == argv[i]) return argv[i + 1];
+    return nullptr;
+  }
+  std::string str(const std::string& k, const std::string& d) const { return raw(k) ? raw(k) : d; }
+  int i(const std::string& k, int d) const { return raw(k) ? atoi(raw(k)) : d; }
+  double f(const std::string& k, double d) const { return raw(k) ? atof(raw(k)) : d; }
+};
+
+double Ms(std::chrono::steady_clock::time_point a) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+}
+
+int Run(int argc, char** argv) {
+  Flags fl{argc, argv};
+  const std::string filename = fl.str("-f", "./data/com-friendster.ungraph.txt.bin");
+  const double mem = fl.f("-m", 10);
+  const int pattern_type = fl.i("-p", 1);
+  const int k_cliques = fl.i("-kc", 0);
+  const int k_motifs = fl.i("-km", 0);
+  const int thread_num = fl.i("-t", 1);
+  const int queue_size = fl.i("-qs", 1);
+  const int producers = fl.i("-pn", 1);
+  const int consumers = fl.i("-cn", 1);
+  const int do_match = fl.i("-dm", 1);
+  const int do_filter = fl.i("-df", 1);
+  const int do_reorder = fl.i("-dr", 0);
+  const int do_split = fl.i("-ds", 0);
+  const int split_times = fl.i("-dst", 1);
+
+  int gpus = 0;
+  vsgm_device_count(&gpus);
+  if (consumers > gpus) {
+    std::cerr << "need " << consumers << " GPUs for -cn, found " << gpus << std::endl;
+    return 2;
+  }
+  if (queue_size < consumers) {
+    std::cerr << "-qs must be >= -cn" << std::endl;
+    return 2;
+  }
+  std::cout << "m: " << mem << " t: " << thread_num << " qs: " << queue_size << " pn: " << producers << " cn: " << consumers << std::endl;
+
+  std::vector<Query> queries;
+  if (k_cliques) queries.push_back(Query::Clique(k_cliques));
+  else if (k_motifs) queries = Query::Graphlets(k_motifs);
+  else queries.push_back(Query::Preset(pattern_type));
+
+  std::vector<std::unique_ptr<Plan>> plans;
+  int hop = 0;
+  uint64_t root_degree = do_filter ? ~0ull : 0;
+  for (auto& q : queries) {
+    std::cout << q.Describe();
+    plans.emplace_back(new Plan(q));
+    plans.back()->Optimize();
+    std::cout << plans.back()->Serialize();
+    hop = std::max(hop, plans.back()->GetHop());
+    root_degree = std::min<uint64_t>(root_degree, plans.back()->GetRootDegree());
+  }
+
+  auto t0 = std::chrono::steady_clock::now();
+  Graph graph = ReadCSRBinFile(filename);
+  std::cout << "read graph " << filename << " V=" << graph.vertex_count << " entries=" << graph.cols.size() << " in " << Ms(t0) << " ms"
+            << std::endl;
+  ViewBinManager vbm(&graph, hop, thread_num);
+  vbm.LoadViewBinPartition(ViewBinPartitionName(filename, hop));
+  if (do_reorder) vbm.Reorder();
+  if (do_split) vbm.Split(consumers, split_times);
+  const uint64_t max_sources = vbm.GetMaxPartitionedSourcesNum();
+  const uint64_t max_view_bytes = (uint64_t)(mem * 1000 * 1000 * 1000);
+
+  std::vector<std::unique_ptr<ViewBinHolder>> holders;
+  for (int i = 0; i < queue_size; ++i) holders.emplace_back(new ViewBinHolder(graph.vertex_count, max_view_bytes));
+  BoundedQueue assigned(queue_size), released(queue_size);
+  for (int i = 0; i < queue_size; ++i) released.push(i);
+
+  std::atomic<int> next_bin{0};
+  std::atomic<int> producers_done{0};
+  std::atomic<bool> failed{false};
+  std::mutex out_mu;
+  auto say = [&](const std::string& s) {
+    std::lock_guard<std::mutex> lk(out_mu);
+    std::cout << s << std::flush;
+  };
+  std::vector<std::thread> threads;
+  const auto& pool = vbm.GetViewBinPool();
+  for (int p = 0; p < producers; ++p)
+    threads.emplace_back([&, p] {
+      try {
+        for (int bin; (bin = next_bin++) < (int)pool.size() && !failed;) {
+          int holder = -1;
+          if (!released.pop(holder)) break;
+          auto t = std::chrono::steady_clock::now();
+          pool[bin]->Materialize(*holders[holder], root_degree);
+          std::ostringstream os;
+          os << "producer " << p << " produces " << bin << " in " << holder << " (" << Ms(t) << " ms)\n";
+          say(os.str());
+          assigned.push(holder);
+        }
+      } catch (const std::exception& ex) {
+        say(std::string("producer failed: ") + ex.what() + "\n");
+        failed = true;
+      }
+      if (++producers_done == producers) assigned.close();
+    });
+
+  std::vector<std::vector<uintC>> counts(consumers, std::vector<uintC>(plans.size(), 0));
+  std::vector<const Plan*> plan_ptrs;
+  for (auto& p : plans) plan_ptrs.push_back(p.get());
+  for (int c = 0; c < consumers; ++c)
+    threads.emplace_back([&, c] {
+      try {
+        PipelineExecutor exec(c, graph.vertex_count, plan_ptrs, max_sources);
+        int holder;
+        while (assigned.pop(holder)) {
+          auto t = std::chrono::steady_clock::now();
+          exec.Transfer(*holders[holder]);
+          std::ostringstream os;
+          os << "consumer " << c << " transfers holder " << holder << " (" << Ms(t) << " ms)\n";
+          say(os.str());
+          released.push(holder);
+          t = std::chrono::steady_clock::now();
+          exec.Match(do_match != 0);
+          std::ostringstream os2;
+          os2 << "consumer " << c << " completes match of holder " << holder << " (" << Ms(t) << " ms)\n";
+          say(os2.str());
+        }
+        counts[c] = exec.GetTotalCounts();
+      } catch (const std::exception& ex) {
+        say(std::string("consumer failed: ") + ex.what() + "\n");
+        failed = true;
+        released.close();
+        assigned.close();
+      }
+    });
+  auto tt = std::chrono::steady_clock::now();
+  for (auto& t : threads) t.join();
+  std::cout << "total time: " << Ms(tt) << " ms" << std::endl;
+  if (failed) return 1;
+  for (size_t i = 0; i < plans.size(); ++i) {
+    uintC total = 0;
+    for (int c = 0; c < consumers; ++c) total += counts[c][i];
+    std::cout << "total count for query " << i << ": " << total << std::endl;
+  }
+  return 0;
+}
+
+}  // namespace
+}  // namespace vsgm
+
+extern "C" int vsgm_pm_pipeline(int argc, char** argv) {
+  try {
+    return vsgm::Run(argc, argv);
+  } catch (const std::exception& ex) {
+    std::cerr << "pm_pipeline: " << ex.what() << std::endl;
+    return 1;
+  }
+}
