@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the VSGM pm_pipeline enumeration path on one MI355X.
+
+Workload (default): seeded R-MAT scale 22, edge factor 16,
+a/b/c/d = .57/.19/.19/.05, made undirected, deduplicated, self loops
+removed, as a u32 CSR; one view bin holding every vertex (a single k-means
+cluster) materialised as a 2-hop view with the root-degree filter; patterns
+4-clique (-kc 4), diamond (-p 2) and 4-cycle (-p 1), heuristic plans, one
+pattern per launch sequence. A step is one Match of all three patterns over
+the view resident in HBM; `value` is embeddings enumerated per second over
+the timed steps.
+
+The JSON line also carries
+  roofline      for the kernel with the most device time in the timed
+                region: algorithmic bytes per launch / average launch time
+                (HIP events on the executor's stream), against 8 TB/s;
+                `traffic` from the rocprof PMC summary in profiles/ when present;
+  cpu_baseline  the oracle's C restatement of the level algorithm on the
+                host cores, over a bounded sample of root vertices.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PATTERNS = {"kc3": ("clique", 3), "kc4": ("clique", 4), "kc5": ("clique", 5)}
+PATTERNS.update({f"p{k}": ("preset", k) for k in range(19)})
+NAMES = {"kc3": "triangle", "kc4": "4-clique", "kc5": "5-clique", "p1": "4-cycle", "p2": "diamond", "p4": "house"}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_query(V, key):
+    kind, k = PATTERNS[key]
+    return V.Query.clique(k) if kind == "clique" else V.Query.preset(k)
+
+
+def cpu_baseline(O, rp, cl, plan_texts, budget_s, threads, seed=12345):
+    """Oracle (C restatement of the level algorithm, OpenMP over roots) on a
+    seeded uniform sample of root vertices, all patterns. The sample grows
+    4x until one pass takes at least budget_s / 4 seconds; the rate of the
+    last pass is reported (embeddings found from the sampled roots per
+    second)."""
+    import numpy as np
+
+    n = len(rp) - 1
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(n).astype(np.uint32)
+    m = 64
+    while True:
+        roots = np.sort(order[: min(m, n)])
+        t = time.perf_counter()
+        total = 0
+        for txt in plan_texts:
+            total += O.count_plan(rp, cl, txt, sources=roots, threads=threads)
+        dt = time.perf_counter() - t
+        log(f"cpu baseline sample {len(roots)} roots: {total} embeddings in {dt:.2f}s")
+        if dt >= budget_s / 4 or m >= n:
+            return {
+                "value": total / dt if dt > 0 else 0.0,
+                "unit": "embeddings/s",
+                "cores": threads,
+                "kind": "port",
+                "sample": f"{len(roots)} uniformly drawn roots (seed {seed}) of {n} vertices, all patterns; "
+                f"{total} embeddings in {dt:.2f} s",
+            }
+        m *= 4
+
+
+def load_traffic(kernel, workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--patterns", default="kc4,p2,p1")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--detail-out", default="")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by view bins "
+              f"across GPUs with no data exchange; only the single-GPU share is built and measured here)", flush=True)
+        sys.exit(2)
+
+    import vsgm_amd as V
+
+    assert V.device_count() > 0, "no GPU visible"
+    keys = [k.strip() for k in args.patterns.split(",") if k.strip()]
+    workload = f"rmat{args.scale}"
+
+    t = time.perf_counter()
+    g = V.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed)
+    gen_s = time.perf_counter() - t
+    log(f"R-MAT s{args.scale} ef{args.edge_factor}: V={g.vertex_count} E={g.edge_count} maxdeg={g.max_degree} "
+        f"generated in {gen_s:.1f}s")
+
+    queries = [make_query(V, k) for k in keys]
+    plans = [V.Plan.optimize(q) for q in queries]
+    hop = max(p.hop for p in plans)
+    root_degree = min(p.root_degree for p in plans)
+
+    # one view bin = all vertices (single cluster), hop-hop view with the root-degree filter
+    t = time.perf_counter()
+    vbm = V.ViewBinManager.from_assignment(g, hop, 1, [0] * g.vertex_count)
+    holder = V.ViewBinHolder(g.vertex_count, max(4, g.entries * 4))
+    vbm.materialize(0, holder, root_degree)
+    view_s = time.perf_counter() - t
+    info = holder.info()
+    log(f"view: hop={hop} root_degree={root_degree} vertices={info['vertices']} entries={info['entries']} in {view_s:.1f}s")
+
+    ex = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
+    t = time.perf_counter()
+    ex.transfer(holder)
+    xfer_s = time.perf_counter() - t
+    log(f"transfer {info['entries'] * 4 / 1e9 + (g.vertex_count + 1) * 8 / 1e9:.2f} GB in {xfer_s:.2f}s")
+
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        ex.match()
+        log(f"warmup {w}: {time.perf_counter() - t:.2f}s counts={ex.total_counts}")
+    ref_counts = [c // max(args.warmup, 1) for c in ex.total_counts] if args.warmup else None
+
+    ex.reset()
+    ex.profile(True)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        ex.match()
+        log(f"step {s}: {time.perf_counter() - ts:.2f}s")
+    t1 = time.perf_counter()
+    kstats = ex.kernel_stats()
+    ex.profile(False)
+
+    counts = [c // args.steps for c in ex.total_counts]
+    assert all(c * args.steps == tc for c, tc in zip(counts, ex.total_counts)), "counts differ between steps"
+    if ref_counts is not None:
+        assert counts == ref_counts, f"timed counts {counts} differ from warmup counts {ref_counts}"
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    match_ms = [m / args.steps for m in ex.match_ms]
+    value = sum(counts) / (ms_per_step / 1000.0)
+    per_pattern = {f"{NAMES.get(k, k)}({k})": {"count": c, "ms": round(m, 3), "embeddings_per_s": c / (m / 1000.0) if m else 0.0}
+                   for k, c, m in zip(keys, counts, match_ms)}
+
+    dom = max(kstats, key=lambda s: s["total_ms"]) if kstats else None
+    roofline = None
+    if dom:
+        per_launch_bytes = dom["algo_bytes"] / dom["launches"]
+        per_launch_s = dom["total_ms"] / dom["launches"] / 1000.0
+        achieved = per_launch_bytes / per_launch_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(dom["name"], workload),
+                    "kernel": dom["name"], "launches": dom["launches"], "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 3),
+                    "algo_bytes_per_launch": int(per_launch_bytes)}
+    for s in sorted(kstats, key=lambda s: -s["total_ms"]):
+        log(f"kernel {s['name']:16s} launches={s['launches']:6d} total={s['total_ms']:10.2f}ms "
+            f"algoGB/s={s['algo_bytes'] / max(s['total_ms'], 1e-9) / 1e6:8.1f} work={s['work']:.3e} paths={s['paths']:.3e}")
+    level_stats = {k: ex.level_stats(i) for i, k in enumerate(keys)}
+    for k in keys:
+        for lvl, st in enumerate(level_stats[k]):
+            log(f"{k} L{lvl} batches={st['batches']} paths={st['paths']} work={st['work']} outputs={st['outputs']}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+
+        rp, cl = g.csr(copy=False)
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(O, rp, cl, [p.serialize() for p in plans], args.cpu_seconds, threads)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "embeddings/sec per pattern, 1 MI355X; learned vs heuristic plan, exact counts",
+        "value": value,
+        "unit": "embeddings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": f"synthetic: seeded R-MAT (seed {args.seed}) generated on the host",
+        "config": {
+            "workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (.57/.19/.19/.05), single-bin {hop}-hop view, "
+                        f"patterns {','.join(keys)}, heuristic plans",
+            "scale": args.scale,
+            "vertices": g.vertex_count,
+            "edges": g.edge_count,
+            "patterns": keys,
+            "counts": counts,
+            "per_pattern": per_pattern,
+            "plan": "heuristic",
+            "graph_gen_s": round(gen_s, 2),
+            "view_build_s": round(view_s, 2),
+            "transfer_s": round(xfer_s, 3),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if args.detail_out:
+        with open(args.detail_out, "w") as f:
+            json.dump({"line": line, "kernels": kstats, "levels": level_stats}, f, indent=1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -99,6 +99,7 @@ typedef struct vsgm_level_stat {
   uint64_t paths;
   uint64_t work;
   uint64_t outputs;
+  uint64_t adj_entries;
   double ms;
 } vsgm_level_stat;
 
@@ -116,6 +117,8 @@ int vsgm_executor_match_ms(const vsgm_executor* e, double* ms, int n);
 int vsgm_executor_reset(vsgm_executor* e);
 int vsgm_executor_level_stats(const vsgm_executor* e, int plan, vsgm_level_stat* out, int cap, int* n_out);
 int vsgm_executor_peak_bytes(const vsgm_executor* e, uint64_t* bytes);
+/* The executor's device context, for the profiling calls of vsgm_hip.h. */
+int vsgm_executor_context(vsgm_executor* e, vsgm_ctx** ctx);
 void vsgm_executor_free(vsgm_executor* e);
 
 /* ---- the pm_pipeline driver (src/app/pattern_matching_pipeline.cu:27-201) */

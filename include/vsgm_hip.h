@@ -145,12 +145,36 @@ typedef struct vsgm_expand_out {
 } vsgm_expand_out;
 
 /* Sizes seen by the last level op (work = candidate-vs-list elements
- * visited after range narrowing, outputs = surviving elements). */
+ * visited after range narrowing, outputs = surviving elements,
+ * adj_entries = sum over paths of the degrees of all backward vertices of
+ * an intersection, i.e. the list volume of the merge model). */
 typedef struct vsgm_op_stats {
   uint64_t paths;
   uint64_t work;
   uint64_t outputs;
+  uint64_t adj_entries;
 } vsgm_op_stats;
+
+/* Per-kernel profile: every level-op kernel launched while profiling is on
+ * is bracketed with HIP events on the context stream. Kernels are named
+ * "<stage>_<op>" with stage setup / map / scatter and op icount (COMPUTE_COUNT),
+ * iwrite (COMPUTE), expand (MATERIALIZE), filter (FILTER_COMPUTE), count2
+ * (two-set COUNT), ipath (per-path counts). algo_bytes sums the launches'
+ * algorithmic HBM bytes (for intersections the merge model: every backward
+ * adjacency list read once, plus each path's bound vertices and row_ptrs). */
+typedef struct vsgm_kernel_stat {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+  double max_ms;
+  uint64_t algo_bytes;
+  uint64_t work;
+  uint64_t paths;
+} vsgm_kernel_stat;
+/* Turn profiling on/off; either way the records collected so far are dropped. */
+int vsgm_profile_enable(vsgm_ctx* ctx, int enable);
+/* Aggregate the records by kernel name (synchronises). */
+int vsgm_profile_collect(vsgm_ctx* ctx, vsgm_kernel_stat* out, int cap, int* n_out);
 
 /* COMPUTE (ComputeGeneral, src/executor/ops/compute.h:7-41 via
  * GpsmBinSearch, src/gpu/gpu_utils_gpsm_bin_search.h:134-171): for each

@@ -60,10 +60,20 @@ PP = C.POINTER(C.c_void_p)
 
 class LevelStat(C.Structure):
     _fields_ = [("batches", C.c_uint64), ("paths", C.c_uint64), ("work", C.c_uint64), ("outputs", C.c_uint64),
-                ("ms", C.c_double)]
+                ("adj_entries", C.c_uint64), ("ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double), ("max_ms", C.c_double),
+                ("algo_bytes", C.c_uint64), ("work", C.c_uint64), ("paths", C.c_uint64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["name"] = self.name.decode()
+        return d
 
 
 def _declare(L):
@@ -115,6 +125,10 @@ def _declare(L):
         "vsgm_executor_reset": ([P], C.c_int),
         "vsgm_executor_level_stats": ([P, C.c_int, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_peak_bytes": ([P, C.POINTER(u64)], C.c_int),
+        "vsgm_executor_context": ([P, PP], C.c_int),
+        "vsgm_profile_enable": ([P, C.c_int], C.c_int),
+        "vsgm_profile_collect": ([P, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "vsgm_last_error": ([P], C.c_char_p),
         "vsgm_executor_free": ([P], None),
         "vsgm_pm_pipeline": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
     }
@@ -479,6 +493,25 @@ class PipelineExecutor(_Handle):
         b = u64()
         _check(lib().vsgm_executor_peak_bytes(self._h, C.byref(b)), "peak")
         return b.value
+
+    def _ctx(self):
+        c = C.c_void_p()
+        _check(lib().vsgm_executor_context(self._h, C.byref(c)), "context")
+        return c.value
+
+    def profile(self, enable: bool = True):
+        """Bracket every level-op kernel launch with HIP events (on the executor's stream)."""
+        ctx = self._ctx()
+        if lib().vsgm_profile_enable(ctx, int(enable)) != 0:
+            raise VsgmError(lib().vsgm_last_error(ctx).decode())
+
+    def kernel_stats(self) -> List[dict]:
+        ctx = self._ctx()
+        arr = (KernelStat * 64)()
+        n = C.c_int()
+        if lib().vsgm_profile_collect(ctx, arr, 64, C.byref(n)) != 0:
+            raise VsgmError(lib().vsgm_last_error(ctx).decode())
+        return [arr[i].as_dict() for i in range(min(n.value, 64))]
 
 
 def pm_pipeline(argv: Sequence[str]) -> int:

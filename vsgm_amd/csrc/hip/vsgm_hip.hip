@@ -276,10 +276,15 @@ __global__ __launch_bounds__(kBlock) void k_fill(uint64_t* d, uint64_t n, uint64
 // level operators
 
 // Per-path segment: begin (low 56 bits, with the pivot position or the
-// count side in the top byte) and length.
+// count side in the top byte) and length. For intersections, *adj_sum
+// accumulates sum over paths of the degrees of all backward vertices (the
+// merge-model list volume used for the algorithmic byte count).
 template <int OP>
-__global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* __restrict__ seg_begin, uint64_t* __restrict__ seg_len) {
+__global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* __restrict__ seg_begin, uint64_t* __restrict__ seg_len,
+                                                  unsigned long long* __restrict__ adj_sum) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
   const uint64_t P = a.path_num;
+  uint64_t deg_sum = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kBlock) {
     uint64_t lo = 0, hi = 0, tag = 0;
     if (is_intersect(OP)) {
@@ -287,6 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* _
       for (uint32_t i = 0; i < a.n_back; ++i) {
         uint32_t v = a.inst[a.back[i]][p];
         uint64_t b = a.row_ptrs[v], e = a.row_ptrs[v + 1];
+        deg_sum += e - b;
         if (i == 0 || e - b < best_deg) {  // first minimum wins (ThreadChoosePivotLevel)
           best_deg = e - b;
           lo = b;
@@ -317,7 +323,15 @@ __global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* _
     seg_begin[p] = lo | (tag << 56);
     seg_len[p] = hi - lo;
   }
+  if (is_intersect(OP)) {
+    uint64_t s = block_sum(deg_sum, s_tmp);
+    if (threadIdx.x == 0 && s) atomicAdd(adj_sum, (unsigned long long)s);
+  }
 }
+
+// out[0] = *total (the scan total), out[1] stays: packs the two values read
+// back after a setup + scan into one 16-byte copy.
+__global__ void k_copy1(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst) { dst[0] = src[0]; }
 
 struct ExpandCols {
   uint32_t n_mat;
@@ -594,8 +608,18 @@ struct vsgm_ctx {
   std::map<void*, uint64_t> blocks;       // live block -> size class
   uint64_t cached_bytes = 0;
   uint64_t* h_pinned = nullptr;           // small pinned staging for D2H of totals
+  uint64_t* d_stats = nullptr;            // device u64[2]: scan total, adjacency degree sum
   std::string err;
-  vsgm_op_stats stats{0, 0, 0};
+  vsgm_op_stats stats{0, 0, 0, 0};
+  // kernel profiling: HIP events around each launch on the context stream
+  struct KernelRecord {
+    const char* name;
+    hipEvent_t start, stop;
+    uint64_t bytes, work, paths;
+  };
+  bool profiling = false;
+  std::vector<KernelRecord> records;
+  std::vector<hipEvent_t> spare_events;
 };
 
 namespace {
@@ -699,6 +723,36 @@ struct Temps {
   }
 };
 
+hipEvent_t take_event(vsgm_ctx* ctx) {
+  hipEvent_t e = nullptr;
+  if (!ctx->spare_events.empty()) {
+    e = ctx->spare_events.back();
+    ctx->spare_events.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipGetLastError();
+    e = nullptr;
+  }
+  return e;
+}
+
+// Brackets one kernel launch with events when profiling is on; `bytes` is
+// the launch's algorithmic byte count.
+struct ProfScope {
+  vsgm_ctx* ctx;
+  long idx = -1;
+  ProfScope(vsgm_ctx* c, const char* name, uint64_t bytes, uint64_t work, uint64_t paths) : ctx(c) {
+    if (!ctx->profiling) return;
+    hipEvent_t a = take_event(ctx), b = take_event(ctx);
+    if (!a || !b) return;
+    (void)hipEventRecord(a, ctx->stream);
+    ctx->records.push_back({name, a, b, bytes, work, paths});
+    idx = (long)ctx->records.size() - 1;
+  }
+  ~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(ctx->records[idx].stop, ctx->stream);
+  }
+};
+
 int read_u64(vsgm_ctx* ctx, const uint64_t* d, uint64_t* h) {
   HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -756,22 +810,74 @@ int check_cand(vsgm_ctx* ctx, const vsgm_level_args* a, uint32_t u) {
   return VSGM_OK;
 }
 
-// Shared driver: setup + scan (+ map) of one level operator.
+// Kernel names used by the profiler, per operator.
+template <int OP>
+constexpr const char* kname(int stage) {
+  // stage 0: setup, 1: map, 2: scatter
+  return OP == OP_ICOUNT   ? (stage == 0 ? "setup_icount" : "map_icount")
+         : OP == OP_IWRITE ? (stage == 0 ? "setup_iwrite" : stage == 1 ? "map_iwrite" : "scatter_iwrite")
+         : OP == OP_EXPAND ? (stage == 0 ? "setup_expand" : stage == 1 ? "map_expand" : "scatter_expand")
+         : OP == OP_FILTER ? (stage == 0 ? "setup_filter" : stage == 1 ? "map_filter" : "scatter_filter")
+         : OP == OP_COUNT2 ? (stage == 0 ? "setup_count2" : "map_count2")
+                           : (stage == 0 ? "setup_ipath" : "map_ipath");
+}
+
+// Algorithmic bytes one path costs the setup kernel: its bound vertices,
+// their row_ptrs (or candidate rows) and the two segment words written.
+template <int OP>
+uint64_t setup_bytes_per_path(const vsgm_level_args& a) {
+  if (is_intersect(OP)) return (4 + 16) * (uint64_t)a.n_back + 16;
+  if (OP == OP_COUNT2) return 2 * (8 + 16) + 16;
+  return 8 + 16 + 16;
+}
+
+// Shared driver: setup + scan of one level operator; returns the element
+// count W and the merge-model adjacency volume S (intersections).
 template <int OP>
 int run_setup_scan(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t** seg_begin, uint64_t** seg_off, uint64_t* W) {
   const uint64_t P = a.path_num;
   VSGM_TRY(t.get(P, seg_begin));
   VSGM_TRY(t.get(P + 1, seg_off));
-  hipLaunchKernelGGL(k_setup<OP>, dim3(grid_for((P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, a, *seg_begin, *seg_off);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stats, 2, 0);
+  {
+    ProfScope ps(ctx, kname<OP>(0), P * setup_bytes_per_path<OP>(a), P, P);
+    hipLaunchKernelGGL(k_setup<OP>, dim3(grid_for((P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, a, *seg_begin, *seg_off,
+                       (unsigned long long*)(ctx->d_stats + 1));
+  }
   VSGM_TRY(launch_check(ctx, "k_setup"));
   VSGM_TRY(scan_impl(ctx, *seg_off, *seg_off, P));
-  VSGM_TRY(read_u64(ctx, *seg_off + P, W));
+  hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, ctx->stream, *seg_off + P, ctx->d_stats);
+  VSGM_TRY(launch_check(ctx, "k_copy1"));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_stats, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *W = ctx->h_pinned[0];
+  ctx->stats.adj_entries = ctx->h_pinned[1];
   return VSGM_OK;
+}
+
+// Algorithmic bytes of the map kernel: intersections read every backward
+// adjacency list once (the merge model: 4 * S), each path's bound vertices
+// and row_ptrs, and its segment words; the other ops stream their W
+// candidate elements.
+template <int OP>
+uint64_t map_bytes(const vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t W, uint64_t n_chunks) {
+  const uint64_t P = a.path_num;
+  uint64_t b;
+  if (is_intersect(OP)) {
+    b = 4 * ctx->stats.adj_entries + P * ((4 + 16) * (uint64_t)a.n_back + 16);
+    if (OP == OP_ICOUNT && a.other >= 0) b += P * (8 + 16);
+  } else if (OP == OP_COUNT2) {
+    b = 4 * W + P * (2 * (8 + 16) + 16);
+  } else {
+    b = 4 * W + P * 16;
+  }
+  if (OP == OP_IWRITE || OP == OP_EXPAND || OP == OP_FILTER) b += (W + 7) / 8 + 8 * n_chunks;
+  return b;
 }
 
 template <int OP>
 int run_count(vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t* d_total, uint64_t* d_path_counts) {
-  ctx->stats = {a.path_num, 0, 0};
+  ctx->stats = {a.path_num, 0, 0, 0};
   if (a.path_num == 0) return VSGM_OK;
   Temps t(ctx);
   uint64_t *seg_begin, *seg_off, W;
@@ -779,18 +885,28 @@ int run_count(vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t* d_total, uint64
   ctx->stats.work = W;
   if (W == 0) return VSGM_OK;
   uint64_t n_chunks = (W + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks,
-                     (uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)d_total, (unsigned long long*)d_path_counts);
+  {
+    ProfScope ps(ctx, kname<OP>(1), map_bytes<OP>(ctx, a, W, n_chunks), W, a.path_num);
+    hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks,
+                       (uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)d_total, (unsigned long long*)d_path_counts);
+  }
   return launch_check(ctx, "k_map(count)");
 }
 
-// Compacting ops: returns the mask-derived chunk offsets and output count.
+// Compacting ops: write the valid elements (count of them: C).
 template <int OP>
 int run_compact(vsgm_ctx* ctx, const vsgm_level_args& a, uint32_t* vals_or_null, uint64_t* out_off_or_null, ExpandCols cols,
                 Temps& t, uint64_t W, uint64_t* seg_begin, uint64_t* seg_off, uint64_t* mask, uint64_t* chunk_off) {
   uint64_t n_chunks = (W + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(k_scatter<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask,
-                     chunk_off, vals_or_null, out_off_or_null, cols);
+  const uint64_t P = a.path_num, C = ctx->stats.outputs;
+  uint64_t bytes = (W + 7) / 8 + 8 * n_chunks + 16 * P + 4 * C;
+  if (OP == OP_EXPAND) bytes += C * (4 * (cols.n_mat + 1) + 8 * cols.n_cu) + C * (4 * cols.n_mat + 8 * cols.n_cu);
+  else bytes += 4 * C + 8 * (P + 1);
+  {
+    ProfScope ps(ctx, kname<OP>(2), bytes, W, P);
+    hipLaunchKernelGGL(k_scatter<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask,
+                       chunk_off, vals_or_null, out_off_or_null, cols);
+  }
   return launch_check(ctx, "k_scatter");
 }
 
@@ -800,17 +916,22 @@ int run_mark(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t W, uint
   uint64_t n_chunks = (W + kChunk - 1) / kChunk;
   VSGM_TRY(t.get((W + kWave - 1) / kWave, mask));
   VSGM_TRY(t.get(n_chunks + 1, chunk_off));
-  hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, *mask,
-                     *chunk_off, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+  {
+    ProfScope ps(ctx, kname<OP>(1), map_bytes<OP>(ctx, a, W, n_chunks), W, a.path_num);
+    hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, *mask,
+                       *chunk_off, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+  }
   VSGM_TRY(launch_check(ctx, "k_map(mark)"));
   VSGM_TRY(scan_impl(ctx, *chunk_off, *chunk_off, n_chunks));
-  return read_u64(ctx, *chunk_off + n_chunks, count);
+  VSGM_TRY(read_u64(ctx, *chunk_off + n_chunks, count));
+  ctx->stats.outputs = *count;
+  return VSGM_OK;
 }
 
 template <int OP>
 int run_csr(vsgm_ctx* ctx, const vsgm_level_args& a, vsgm_csr_out* out) {
   const uint64_t P = a.path_num;
-  ctx->stats = {P, 0, 0};
+  ctx->stats = {P, 0, 0, 0};
   out->offsets = nullptr;
   out->values = nullptr;
   out->count = 0;
@@ -886,6 +1007,7 @@ int vsgm_ctx_create(int device, uint64_t mem_budget, vsgm_ctx** out) {
   for (int i = 0; i < VSGM_TIMER_SLOTS; ++i)
     if (hipEventCreate(&ctx->ev_start[i]) != hipSuccess || hipEventCreate(&ctx->ev_stop[i]) != hipSuccess) return bail(VSGM_ERR_HIP);
   if (hipHostMalloc((void**)&ctx->h_pinned, 64, hipHostMallocDefault) != hipSuccess) return bail(VSGM_ERR_HIP);
+  if (hipMalloc((void**)&ctx->d_stats, 2 * sizeof(uint64_t)) != hipSuccess) return bail(VSGM_ERR_HIP);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bail(VSGM_ERR_HIP);
   ctx->budget = mem_budget ? mem_budget : (uint64_t)(0.9 * (double)free_b);
@@ -900,6 +1022,12 @@ int vsgm_ctx_destroy(vsgm_ctx* ctx) {
   for (auto& kv : ctx->cached) (void)hipFree(kv.second);
   for (auto& kv : ctx->blocks) (void)hipFree(kv.first);
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+  if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+  for (auto& r : ctx->records) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  for (auto e : ctx->spare_events) (void)hipEventDestroy(e);
   for (int i = 0; i < VSGM_TIMER_SLOTS; ++i) {
     if (ctx->ev_start[i]) (void)hipEventDestroy(ctx->ev_start[i]);
     if (ctx->ev_stop[i]) (void)hipEventDestroy(ctx->ev_stop[i]);
@@ -1061,7 +1189,7 @@ int vsgm_expand(vsgm_ctx* ctx, const vsgm_level_args* a, const uint32_t* mat, ui
   }
   memset(out, 0, sizeof(*out));
   const uint64_t P = a->path_num;
-  ctx->stats = {P, 0, 0};
+  ctx->stats = {P, 0, 0, 0};
   Temps t(ctx);
   uint64_t *seg_begin = nullptr, *seg_off = nullptr, W = 0;
   if (P) VSGM_TRY(run_setup_scan<OP_EXPAND>(ctx, *a, t, &seg_begin, &seg_off, &W));
@@ -1114,7 +1242,7 @@ int vsgm_count_sets(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_total) 
   if (!d_total) return fail(ctx, VSGM_ERR_ARG, "null total");
   VSGM_TRY(check_cand(ctx, a, a->target));
   if (a->other < 0) {
-    ctx->stats = {a->path_num, a->path_num, 0};
+    ctx->stats = {a->path_num, a->path_num, 0, 0};
     if (a->path_num == 0) return VSGM_OK;
     hipLaunchKernelGGL(k_count1, dim3(grid_for((a->path_num + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, *a,
                        (unsigned long long*)d_total);
@@ -1170,6 +1298,45 @@ int vsgm_split_batch(vsgm_ctx* ctx, const uint64_t* d_children, uint64_t n, uint
   *n_batches = ends.size();
   if (!h_ends || ends.size() > max_batches) return fail(ctx, VSGM_ERR_ARG, "split needs %llu batch slots", (unsigned long long)ends.size());
   std::copy(ends.begin(), ends.end(), h_ends);
+  return VSGM_OK;
+}
+
+int vsgm_profile_enable(vsgm_ctx* ctx, int enable) {
+  if (!ctx) return VSGM_ERR_ARG;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->records) {
+    ctx->spare_events.push_back(r.start);
+    ctx->spare_events.push_back(r.stop);
+  }
+  ctx->records.clear();
+  ctx->profiling = enable != 0;
+  return VSGM_OK;
+}
+
+int vsgm_profile_collect(vsgm_ctx* ctx, vsgm_kernel_stat* out, int cap, int* n_out) {
+  if (!ctx || !n_out) return VSGM_ERR_ARG;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<vsgm_kernel_stat> agg;
+  for (auto& r : ctx->records) {
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, r.start, r.stop));
+    auto it = std::find_if(agg.begin(), agg.end(), [&](const vsgm_kernel_stat& s) { return strcmp(s.name, r.name) == 0; });
+    if (it == agg.end()) {
+      vsgm_kernel_stat s;
+      memset(&s, 0, sizeof(s));
+      strncpy(s.name, r.name, sizeof(s.name) - 1);
+      agg.push_back(s);
+      it = agg.end() - 1;
+    }
+    it->launches += 1;
+    it->total_ms += ms;
+    it->max_ms = std::max(it->max_ms, (double)ms);
+    it->algo_bytes += r.bytes;
+    it->work += r.work;
+    it->paths += r.paths;
+  }
+  *n_out = (int)agg.size();
+  for (int i = 0; i < (int)agg.size() && i < cap; ++i) out[i] = agg[i];
   return VSGM_OK;
 }
 

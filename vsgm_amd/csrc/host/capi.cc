@@ -359,8 +359,12 @@ int vsgm_executor_level_stats(const vsgm_executor* e, int plan, vsgm_level_stat*
     VSGM_CHECK(plan >= 0 && plan < (int)all.size(), "plan index out of range");
     const auto& s = all[plan];
     *n_out = (int)s.size();
-    for (int i = 0; i < (int)s.size() && i < cap; ++i) out[i] = vsgm_level_stat{s[i].batches, s[i].paths, s[i].work, s[i].outputs, s[i].ms};
+    for (int i = 0; i < (int)s.size() && i < cap; ++i) out[i] = vsgm_level_stat{s[i].batches, s[i].paths, s[i].work, s[i].outputs, s[i].adj_entries, s[i].ms};
   });
+}
+
+int vsgm_executor_context(vsgm_executor* e, vsgm_ctx** ctx) {
+  return Guard([&] { *ctx = e->e->Context(); });
 }
 
 int vsgm_executor_peak_bytes(const vsgm_executor* e, uint64_t* bytes) {

@@ -399,6 +399,7 @@ void PipelineExecutor::RunBatch(size_t level, const Level& slice) {
   stat.paths += slice.path_num;
   stat.work += st.work;
   stat.outputs += st.outputs;
+  stat.adj_entries += st.adj_entries;
   if (opt_.profile) {
     float ms = 0;
     Check(vsgm_timer_stop(ctx_, 1, &ms), "timer");

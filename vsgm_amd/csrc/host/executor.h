@@ -27,6 +27,7 @@ struct LevelStat {
   uint64_t paths = 0;    // partial embeddings entering the level
   uint64_t work = 0;     // elements visited by the level's kernels
   uint64_t outputs = 0;  // candidates / new paths produced
+  uint64_t adj_entries = 0;  // backward adjacency volume of intersections
   double ms = 0;         // device time (profiling mode only)
 };
 

@@ -1,0 +1,208 @@
+"""ctypes bindings of the device library's C-ABI (include/vsgm_hip.h).
+
+DeviceContext owns a vsgm_ctx (device, stream, memory pool); DeviceArray
+is a pool allocation with numpy upload/download. The level operators are
+exposed with the vsgm_level_args structure of the header.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from .api import VsgmError, library_paths
+
+MAXQ = 7
+_hip = None
+
+
+def hiplib():
+    global _hip
+    if _hip is None:
+        from .api import lib
+
+        lib()  # loads libvsgm_hip.so globally first
+        _hip = C.CDLL(library_paths()[0], mode=C.RTLD_GLOBAL)
+        _declare(_hip)
+    return _hip
+
+
+class Cond(C.Structure):
+    _fields_ = [("op", C.c_uint32), ("operand", C.c_uint32)]
+
+
+class LevelArgs(C.Structure):
+    _fields_ = [
+        ("row_ptrs", C.c_void_p),
+        ("cols", C.c_void_p),
+        ("vertex_count", C.c_uint64),
+        ("path_num", C.c_uint64),
+        ("inst", C.c_void_p * MAXQ),
+        ("cand", C.c_void_p * MAXQ),
+        ("cand_off", C.c_void_p * MAXQ),
+        ("cand_idx", C.c_void_p * MAXQ),
+        ("cand_idx_base", C.c_uint64 * MAXQ),
+        ("target", C.c_uint32),
+        ("n_back", C.c_uint32),
+        ("back", C.c_uint32 * MAXQ),
+        ("n_cond", C.c_uint32),
+        ("cond", Cond * MAXQ),
+        ("other", C.c_int32),
+        ("other_op", C.c_uint32),
+    ]
+
+
+class CsrOut(C.Structure):
+    _fields_ = [("offsets", C.c_void_p), ("values", C.c_void_p), ("count", C.c_uint64)]
+
+
+class ExpandOut(C.Structure):
+    _fields_ = [("path_num", C.c_uint64), ("inst", C.c_void_p * MAXQ), ("cand_idx", C.c_void_p * MAXQ)]
+
+
+class OpStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("work", C.c_uint64), ("outputs", C.c_uint64), ("adj_entries", C.c_uint64)]
+
+
+def _declare(L):
+    P, u64, i = C.c_void_p, C.c_uint64, C.c_int
+    sig = {
+        "vsgm_ctx_create": ([i, u64, C.POINTER(P)], i),
+        "vsgm_ctx_destroy": ([P], i),
+        "vsgm_ctx_sync": ([P], i),
+        "vsgm_last_error": ([P], C.c_char_p),
+        "vsgm_mem_info": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], i),
+        "vsgm_alloc": ([P, u64, C.POINTER(P)], i),
+        "vsgm_free": ([P, P], i),
+        "vsgm_h2d": ([P, P, P, u64], i),
+        "vsgm_d2h": ([P, P, P, u64], i),
+        "vsgm_scan_excl_u64": ([P, P, P, u64], i),
+        "vsgm_reduce_u64": ([P, P, u64, P], i),
+        "vsgm_gather_u32": ([P, P, u64, P, P], i),
+        "vsgm_gather_u64": ([P, P, u64, P, P], i),
+        "vsgm_iota_u64": ([P, P, u64, u64], i),
+        "vsgm_fill_u64": ([P, P, u64, u64], i),
+        "vsgm_intersect_write": ([P, C.POINTER(LevelArgs), C.POINTER(CsrOut)], i),
+        "vsgm_intersect_count_total": ([P, C.POINTER(LevelArgs), P], i),
+        "vsgm_intersect_count_paths": ([P, C.POINTER(LevelArgs), P], i),
+        "vsgm_expand": ([P, C.POINTER(LevelArgs), P, C.c_uint32, P, C.c_uint32, C.POINTER(ExpandOut)], i),
+        "vsgm_filter": ([P, C.POINTER(LevelArgs), C.POINTER(CsrOut)], i),
+        "vsgm_count_sets": ([P, C.POINTER(LevelArgs), P], i),
+        "vsgm_children_count": ([P, C.POINTER(LevelArgs), i, P], i),
+        "vsgm_split_batch": ([P, P, u64, u64, u64, u64, P, u64, C.POINTER(u64)], i),
+        "vsgm_last_op_stats": ([P, C.POINTER(OpStats)], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+
+class DeviceArray:
+    def __init__(self, ctx: "DeviceContext", ptr: int, n: int, dtype, owned: bool = True):
+        self.ctx, self.ptr, self.n, self.dtype, self.owned = ctx, ptr, n, np.dtype(dtype), owned
+
+    def numpy(self) -> np.ndarray:
+        out = np.zeros(self.n, self.dtype)
+        if self.n:
+            self.ctx.check(hiplib().vsgm_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.owned and self.ptr:
+            hiplib().vsgm_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class DeviceContext:
+    def __init__(self, device: int = 0, mem_budget: int = 0):
+        h = C.c_void_p()
+        s = hiplib().vsgm_ctx_create(device, mem_budget, C.byref(h))
+        if s != 0:
+            raise VsgmError(f"vsgm_ctx_create failed with status {s}")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            hiplib().vsgm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, status, what=""):
+        if status != 0:
+            raise VsgmError(f"{what}: status {status}: {hiplib().vsgm_last_error(self.h).decode()}")
+
+    def empty(self, n: int, dtype) -> DeviceArray:
+        dt = np.dtype(dtype)
+        p = C.c_void_p()
+        self.check(hiplib().vsgm_alloc(self.h, max(n, 1) * dt.itemsize, C.byref(p)), "alloc")
+        return DeviceArray(self, p.value, n, dt)
+
+    def array(self, a) -> DeviceArray:
+        a = np.ascontiguousarray(a)
+        d = self.empty(a.size, a.dtype)
+        if a.size:
+            self.check(hiplib().vsgm_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes), "h2d")
+            self.sync()
+        return d
+
+    def adopt(self, ptr: int, n: int, dtype) -> DeviceArray:
+        return DeviceArray(self, ptr, n, dtype, owned=True)
+
+    def sync(self):
+        self.check(hiplib().vsgm_ctx_sync(self.h), "sync")
+
+    def mem_info(self):
+        a, b, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.check(hiplib().vsgm_mem_info(self.h, C.byref(a), C.byref(b), C.byref(p)), "mem_info")
+        return a.value, b.value, p.value
+
+    # primitives
+    def scan(self, d_in: DeviceArray, d_out: Optional[DeviceArray] = None) -> DeviceArray:
+        out = d_out or self.empty(d_in.n + 1, np.uint64)
+        self.check(hiplib().vsgm_scan_excl_u64(self.h, d_in.ptr, out.ptr, d_in.n), "scan")
+        return out
+
+    def reduce(self, d_in: DeviceArray) -> int:
+        tot = self.array(np.zeros(1, np.uint64))
+        self.check(hiplib().vsgm_reduce_u64(self.h, d_in.ptr, d_in.n, tot.ptr), "reduce")
+        return int(tot.numpy()[0])
+
+    def gather(self, idx: DeviceArray, src: DeviceArray) -> DeviceArray:
+        out = self.empty(idx.n, src.dtype)
+        fn = hiplib().vsgm_gather_u32 if src.dtype == np.uint32 else hiplib().vsgm_gather_u64
+        self.check(fn(self.h, idx.ptr, idx.n, src.ptr, out.ptr), "gather")
+        return out
+
+    def iota(self, n: int, base: int = 0) -> DeviceArray:
+        out = self.empty(n, np.uint64)
+        self.check(hiplib().vsgm_iota_u64(self.h, out.ptr, n, base), "iota")
+        return out
+
+    def fill(self, n: int, value: int) -> DeviceArray:
+        out = self.empty(n, np.uint64)
+        self.check(hiplib().vsgm_fill_u64(self.h, out.ptr, n, value), "fill")
+        return out
+
+    def split_batch(self, children: DeviceArray, pf: int, cf: int, batch_bytes: int):
+        cap = 1 << 16
+        ends = np.zeros(cap, np.uint64)
+        n = C.c_uint64()
+        self.check(hiplib().vsgm_split_batch(self.h, children.ptr, children.n, pf, cf, batch_bytes, ends.ctypes.data, cap,
+                                             C.byref(n)), "split")
+        return ends[: n.value].copy()
+
+    def last_stats(self) -> OpStats:
+        s = OpStats()
+        self.check(hiplib().vsgm_last_op_stats(self.h, C.byref(s)), "stats")
+        return s
