@@ -106,7 +106,8 @@ typedef struct vsgm_level_stat {
 int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* const* plans, int n_plans, uint64_t max_sources,
                          uint64_t mem_budget, vsgm_executor** out);
 /* Options: "batch_bytes" (0 = budget / remaining levels), "min_batch_bytes",
- * "profile" (1 = time every level op). */
+ * "profile" (1 = time every level op), and the device tuning knobs
+ * "isect_kernel" and "max_grid" (vsgm_tuning in vsgm_hip.h). */
 int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value);
 int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h);
 /* Whole graph as one view; sources NULL = every vertex. */

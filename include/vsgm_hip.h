@@ -171,6 +171,23 @@ typedef struct vsgm_kernel_stat {
   uint64_t work;
   uint64_t paths;
 } vsgm_kernel_stat;
+/* Launch-shape knobs of the level kernels, chosen by the autotuner:
+ *   isect_kernel  0: every lane verifies its own candidate with its own
+ *                    binary searches (block-strided elements);
+ *                 1: a wavefront walks a contiguous element run; runs inside
+ *                    one path stream the other lists through LDS tiles from
+ *                    a per-wave cursor, other runs use branchless searches;
+ *                 2: as 0, and each workgroup keeps in LDS one backward
+ *                    adjacency list (<= 4096 entries) bound to the same vertex
+ *                    across its chunk and searches it there;
+ *   max_grid      cap on workgroups per map launch (grid-stride beyond). */
+typedef struct vsgm_tuning {
+  uint32_t isect_kernel;
+  uint32_t max_grid;
+} vsgm_tuning;
+int vsgm_set_tuning(vsgm_ctx* ctx, const vsgm_tuning* t);
+int vsgm_get_tuning(const vsgm_ctx* ctx, vsgm_tuning* t);
+
 /* Turn profiling on/off; either way the records collected so far are dropped. */
 int vsgm_profile_enable(vsgm_ctx* ctx, int enable);
 /* Aggregate the records by kernel name (synchronises). */

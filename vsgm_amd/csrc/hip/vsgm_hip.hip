@@ -47,7 +47,7 @@ constexpr int kMaxGrid = 8192;
 
 enum OpKind { OP_ICOUNT = 0, OP_IWRITE = 1, OP_EXPAND = 2, OP_FILTER = 3, OP_COUNT2 = 4, OP_IPATH = 5 };
 
-__host__ __device__ inline bool is_intersect(int op) { return op == OP_ICOUNT || op == OP_IWRITE || op == OP_IPATH; }
+__host__ __device__ constexpr bool is_intersect(int op) { return op == OP_ICOUNT || op == OP_IWRITE || op == OP_IPATH; }
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -129,6 +129,123 @@ __device__ inline uint64_t count_against_mirror(const uint32_t* s, uint32_t n, u
   if (op == VSGM_LESS_THAN) return pos;               // y < x
   if (op == VSGM_LARGER_THAN) return n - (pos + eq);  // y > x
   return n - eq;
+}
+
+// Wavefront-scope barrier for LDS data exchanged between the lanes of one
+// wavefront (the waves of a workgroup diverge here, so no __syncthreads).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr uint32_t kTile = 256;          // LDS tile of one wavefront (u32 entries)
+constexpr int kCursorTiles = 4;          // tiles streamed from a cursor before searching HBM
+constexpr uint32_t kStageCap = 4096;     // adjacency entries a workgroup can hold in LDS
+constexpr int kCursorSlots = VSGM_MAX_QV + 1;
+
+// Largest power of two <= n (0 for n == 0).
+__device__ inline uint32_t top_step(uint32_t n) { return n ? (1u << (31 - __clz(n))) : 0u; }
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+  for (int d = kWave / 2; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d));
+  return v;
+}
+
+__device__ inline uint64_t readfirstlane_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Binary lifting over a sorted a[0..n): a fixed sequence of steps top, top/2,
+// ..., 1 with predicated probes, so lanes with different n and keys share one
+// instruction stream (no divergent search loops). `top` is a wave-uniform
+// power of two >= top_step of every lane's n; a[0] must be readable even
+// when n == 0. Returns the number of entries < key (lower bound).
+template <typename T>
+__device__ inline uint32_t lift_lower_bound(const T* a, uint32_t n, T key, uint32_t top) {
+  uint32_t pos = 0;
+  for (uint32_t step = top; step; step >>= 1) {
+    const uint32_t probe = pos + step;
+    const bool in = probe <= n;
+    const T v = a[in ? probe - 1 : 0];
+    pos = (in && v < key) ? probe : pos;
+  }
+  return pos;
+}
+
+// Same, returning the number of entries <= key (upper bound).
+template <typename T>
+__device__ inline uint32_t lift_upper_bound(const T* a, uint32_t n, T key, uint32_t top) {
+  uint32_t pos = 0;
+  for (uint32_t step = top; step; step >>= 1) {
+    const uint32_t probe = pos + step;
+    const bool in = probe <= n;
+    const T v = a[in ? probe - 1 : 0];
+    pos = (in && v <= key) ? probe : pos;
+  }
+  return pos;
+}
+
+// Per-lane lower bound in per-lane lists (lanes may hold different lists):
+// binary lifting with the wave's largest list length. `safe` is any readable
+// u32 used as the base of empty lists.
+__device__ inline uint32_t lane_lower_bound(const uint32_t* L, uint32_t n, uint32_t key, const uint32_t* safe, bool& found) {
+  const uint32_t top = top_step(wave_max_u32(n));
+  const uint32_t* base = n ? L : safe;
+  const uint32_t pos = lift_lower_bound(base, n, key, top);
+  const uint32_t v = base[pos < n ? pos : 0];
+  found = pos < n && v == key;
+  return pos;
+}
+
+// lower_bound of every lane's key in L[0..n) for a wavefront whose 64 keys
+// ascend with the lane index, given a wave-uniform cursor <= lower_bound of
+// the smallest key (the previous iteration of the same path left it at the
+// lower_bound of its largest key). Tiles of L are streamed from the cursor
+// through the wave's LDS tile and searched there until every lane has its
+// answer; lanes still unplaced after kCursorTiles tiles (or when the first
+// tile shows the cursor far behind) search the rest of L in HBM. Advances
+// the cursor to the answer of lane 63.
+__device__ inline uint32_t wave_lower_bound_cursor(const uint32_t* __restrict__ L, uint32_t n, uint32_t key, uint32_t& cursor, uint32_t* tile,
+                                                   bool& found) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t kmin = __shfl(key, 0);
+  uint32_t t = cursor;
+  uint32_t pos = n;
+  bool placed = false;
+  found = false;
+  for (int k = 0; k < kCursorTiles && t < n; ++k) {
+    const uint32_t m = min(kTile, n - t);
+#pragma unroll
+    for (uint32_t j = 0; j < kTile / kWave; ++j) {
+      const uint32_t i = lane + j * kWave;
+      if (i < m) tile[i] = L[t + i];
+    }
+    wave_sync();
+    const uint32_t last = tile[m - 1];
+    const bool far = (k == 0) && kmin > last && (n - t > m);
+    if (!far) {
+      const uint32_t j = lift_lower_bound(tile, m, key, kTile);
+      const bool here = !placed && key <= last;  // then j < m
+      pos = here ? t + j : pos;
+      found = here ? (tile[j < m ? j : 0] == key) : found;
+      placed = placed || here;
+    }
+    wave_sync();
+    t += m;
+    if (far || __shfl((int)placed, kWave - 1)) break;
+  }
+  if (t < n && __ballot(!placed) != 0) {
+    // lanes past the streamed tiles: binary lifting over the rest in HBM
+    const uint32_t r = t + lift_lower_bound(L + t, n - t, key, top_step(n - t));
+    const uint32_t rv = L[r < n ? r : n - 1];
+    found = placed ? found : (r < n && rv == key);
+    pos = placed ? pos : r;
+  }
+  cursor = __shfl(pos, kWave - 1);
+  return pos;
 }
 
 // Narrow [lo, hi) of the sorted array arr with the '<' / '>' conditions.
@@ -352,12 +469,19 @@ __device__ inline const uint32_t* seg_source(const vsgm_level_args& a, uint64_t 
 // Verify element c of path p. Returns its contribution (count ops) and sets
 // valid.
 template <int OP>
-__device__ inline uint64_t eval_element(const vsgm_level_args& a, uint64_t p, uint64_t tag, uint32_t c, bool& valid) {
+__device__ inline uint64_t eval_element(const vsgm_level_args& a, uint64_t p, uint64_t tag, uint32_t c, bool& valid,
+                                        uint32_t stage_v = ~0u, const uint32_t* s_list = nullptr, uint32_t stage_n = 0) {
   if (is_intersect(OP)) {
     bool ok = true;
     for (uint32_t i = 0; i < a.n_back && ok; ++i) {
       if (i == tag) continue;
       uint32_t v = a.inst[a.back[i]][p];
+      if (v == stage_v) {
+        // the workgroup holds this adjacency list in LDS
+        uint32_t pos = lower_bound_u32(s_list, stage_n, c);
+        ok = pos < stage_n && s_list[pos] == c;
+        continue;
+      }
       uint64_t b = a.row_ptrs[v];
       uint32_t n = (uint32_t)(a.row_ptrs[v + 1] - b);
       uint32_t pos = lower_bound_u32(a.cols + b, n, c);
@@ -389,6 +513,115 @@ __device__ inline uint64_t eval_element(const vsgm_level_args& a, uint64_t p, ui
       uint32_t n = (uint32_t)(a.cand_off[a.target][r + 1] - b);
       return count_against_mirror(a.cand[a.target] + b, n, c, a.other_op);
     }
+  }
+}
+
+// Same verification as eval_element for a wavefront whose 64 lanes all hold
+// consecutive candidates of one path: every list lookup goes through
+// wave_lower_bound (LDS-staged windows of the other lists).
+template <int OP>
+__device__ inline uint64_t eval_element_wave(const vsgm_level_args& a, uint64_t p, uint64_t tag, uint32_t c, bool& valid, uint32_t* tile,
+                                             uint32_t* cursor) {
+  if (is_intersect(OP)) {
+    bool ok = true;
+    for (uint32_t i = 0; i < a.n_back; ++i) {
+      if (i == tag) continue;
+      if (__ballot(ok) == 0) break;
+      const uint32_t v = a.inst[a.back[i]][p];
+      const uint64_t b = a.row_ptrs[v];
+      const uint32_t n = (uint32_t)(a.row_ptrs[v + 1] - b);
+      bool f;
+      wave_lower_bound_cursor(a.cols + b, n, c, cursor[i], tile, f);
+      ok = ok && f;
+    }
+    ok = ok && non_equal_ok(a, p, c);
+    valid = ok;
+    if (OP == OP_ICOUNT && a.other >= 0) {
+      if (__ballot(ok) == 0) return 0;
+      const uint64_t r = cand_row(a, (uint32_t)a.other, p);
+      const uint64_t b = a.cand_off[a.other][r];
+      const uint32_t n = (uint32_t)(a.cand_off[a.other][r + 1] - b);
+      bool f;
+      const uint32_t pos = wave_lower_bound_cursor(a.cand[a.other] + b, n, c, cursor[VSGM_MAX_QV], tile, f);
+      if (!ok) return 0;
+      const uint32_t eq = f ? 1u : 0u;
+      if (a.other_op == VSGM_LESS_THAN) return n - (pos + eq);
+      if (a.other_op == VSGM_LARGER_THAN) return pos;
+      return n - eq;
+    }
+    return ok ? 1 : 0;
+  } else {  // OP_COUNT2
+    valid = true;
+    const uint32_t u = tag == 0 ? (uint32_t)a.other : a.target;
+    const uint64_t r = cand_row(a, u, p);
+    const uint64_t b = a.cand_off[u][r];
+    const uint32_t n = (uint32_t)(a.cand_off[u][r + 1] - b);
+    bool f;
+    const uint32_t pos = wave_lower_bound_cursor(a.cand[u] + b, n, c, cursor[0], tile, f);
+    const uint32_t eq = f ? 1u : 0u;
+    if (tag == 0) {  // c from the target, counted against the other: "c op y"
+      if (a.other_op == VSGM_LESS_THAN) return n - (pos + eq);
+      if (a.other_op == VSGM_LARGER_THAN) return pos;
+    } else {  // c from the other, counted against the target: "y op c"
+      if (a.other_op == VSGM_LESS_THAN) return pos;
+      if (a.other_op == VSGM_LARGER_THAN) return n - (pos + eq);
+    }
+    return n - eq;
+  }
+}
+
+// Verification for a wavefront whose lanes hold different paths. Every lane
+// executes it (inactive lanes pass active = false and a readable element);
+// all list searches are binary lifting with the wave's longest list, so the
+// lanes never diverge in a search loop.
+template <int OP>
+__device__ inline uint64_t eval_lanes(const vsgm_level_args& a, uint64_t p, uint32_t tag, uint32_t c, bool active, bool& valid) {
+  if (is_intersect(OP)) {
+    bool ok = active;
+    for (uint32_t i = 0; i < a.n_back; ++i) {
+      if (__ballot(ok) == 0) break;
+      const bool skip = (i == tag);
+      const uint32_t v = a.inst[a.back[i]][p];
+      const uint64_t b = a.row_ptrs[v];
+      const uint32_t n = (ok && !skip) ? (uint32_t)(a.row_ptrs[v + 1] - b) : 0u;
+      bool f;
+      lane_lower_bound(a.cols + b, n, c, a.cols, f);
+      ok = skip ? ok : (ok && f);
+    }
+    ok = ok && non_equal_ok(a, p, c);
+    valid = ok;
+    if (OP == OP_ICOUNT && a.other >= 0) {
+      if (__ballot(ok) == 0) return 0;
+      const uint64_t r = cand_row(a, (uint32_t)a.other, p);
+      const uint64_t b = a.cand_off[a.other][r];
+      const uint32_t n_all = (uint32_t)(a.cand_off[a.other][r + 1] - b);
+      bool f;
+      const uint32_t pos = lane_lower_bound(a.cand[a.other] + b, ok ? n_all : 0u, c, a.cand[a.other], f);
+      if (!ok) return 0;
+      const uint32_t eq = f ? 1u : 0u;
+      if (a.other_op == VSGM_LESS_THAN) return n_all - (pos + eq);
+      if (a.other_op == VSGM_LARGER_THAN) return pos;
+      return n_all - eq;
+    }
+    return ok ? 1 : 0;
+  } else {  // OP_COUNT2
+    valid = active;
+    const uint32_t u = tag == 0 ? (uint32_t)a.other : a.target;
+    const uint64_t r = cand_row(a, u, p);
+    const uint64_t b = a.cand_off[u][r];
+    const uint32_t n_all = (uint32_t)(a.cand_off[u][r + 1] - b);
+    bool f;
+    const uint32_t pos = lane_lower_bound(a.cand[u] + b, active ? n_all : 0u, c, a.cand[u], f);
+    if (!active) return 0;
+    const uint32_t eq = f ? 1u : 0u;
+    if (tag == 0) {
+      if (a.other_op == VSGM_LESS_THAN) return n_all - (pos + eq);
+      if (a.other_op == VSGM_LARGER_THAN) return pos;
+    } else {
+      if (a.other_op == VSGM_LESS_THAN) return pos;
+      if (a.other_op == VSGM_LARGER_THAN) return n_all - (pos + eq);
+    }
+    return n_all - eq;
   }
 }
 
@@ -427,36 +660,156 @@ __device__ inline uint64_t path_of(uint64_t e, const ChunkPaths& cp, const uint6
   return cp.first + i;
 }
 
+// Element -> path map of one chunk, built once per chunk in LDS: every path
+// whose non-empty segment starts inside the chunk writes its index at its
+// start position, and an inclusive max-scan over the chunk fills the gaps,
+// so each element finds its path with a single LDS read (load-balance
+// search by scatter + scan). s_rel[i] = segment start of path first+i
+// relative to the chunk start (negative for the path that began earlier).
+struct ChunkMap {
+  uint64_t first;    // path of the chunk's first element
+  uint64_t last;     // path of the chunk's last element
+  uint32_t mapped;   // 1: s_pidx / s_rel valid; 0: more paths than kPathCap
+};
+
+__device__ inline void build_chunk_map(const uint64_t* __restrict__ seg_off, uint64_t P, uint64_t e0, uint64_t e1, ChunkMap& cm,
+                                       int32_t* s_rel, uint16_t* s_pidx, uint32_t* s_scan, ChunkMap* s_cm) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  if (threadIdx.x == 0) {
+    ChunkMap c;
+    c.first = last_le_u64(seg_off, P + 1, e0);
+    c.last = last_le_u64(seg_off, P + 1, e1 - 1);
+    c.mapped = (c.last - c.first + 1) <= (uint64_t)kPathCap ? 1u : 0u;
+    *s_cm = c;
+  }
+  __syncthreads();
+  cm = *s_cm;
+  if (!cm.mapped) return;
+  const uint32_t n_paths = (uint32_t)(cm.last - cm.first + 1);
+  const uint32_t len = (uint32_t)(e1 - e0);
+  for (uint32_t i = threadIdx.x; i <= n_paths; i += kBlock) s_rel[i] = (int32_t)((int64_t)seg_off[cm.first + i] - (int64_t)e0);
+  for (uint32_t j = threadIdx.x; j < (uint32_t)kChunk; j += kBlock) s_pidx[j] = 0;
+  __syncthreads();
+  // paths 1..n_paths-1 start inside the chunk (s_rel >= 1); empty ones write nothing
+  for (uint32_t i = 1 + threadIdx.x; i < n_paths; i += kBlock) {
+    const int32_t s = s_rel[i];
+    if (s_rel[i + 1] > s) s_pidx[s] = (uint16_t)i;
+  }
+  __syncthreads();
+  // inclusive max-scan; thread t owns entries [8t, 8t + 8)
+  uint32_t loc[kItems];
+  uint32_t m = 0;
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t idx = threadIdx.x * kItems + k;
+    m = max(m, idx < len ? (uint32_t)s_pidx[idx] : 0u);
+    loc[k] = m;
+  }
+  uint32_t x = m;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x = max(x, y);
+  }
+  if (lane == kWave - 1) s_scan[wave] = x;
+  __syncthreads();
+  uint32_t before = __shfl_up(x, 1);
+  before = lane ? before : 0u;
+  for (int w = 0; w < wave; ++w) before = max(before, s_scan[w]);
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t idx = threadIdx.x * kItems + k;
+    if (idx < len) s_pidx[idx] = (uint16_t)max(before, loc[k]);
+  }
+  __syncthreads();
+}
+
+// Path of element e in [e0, e1) and the element's rank inside its segment.
+__device__ inline uint64_t chunk_path(uint64_t e, uint64_t e0, const ChunkMap& cm, const int32_t* s_rel, const uint16_t* s_pidx,
+                                      const uint64_t* __restrict__ seg_off, uint64_t& rank) {
+  if (cm.mapped) {
+    const uint32_t j = (uint32_t)(e - e0);
+    const uint32_t i = s_pidx[j];
+    rank = (uint64_t)((int64_t)j - (int64_t)s_rel[i]);
+    return cm.first + i;
+  }
+  const uint64_t i = last_le_u64(seg_off + cm.first, cm.last - cm.first + 2, e);
+  rank = e - seg_off[cm.first + i];
+  return cm.first + i;
+}
+
 // Map: count (OP_ICOUNT, OP_COUNT2, OP_IPATH) or mark valid elements
-// (OP_IWRITE, OP_EXPAND, OP_FILTER).
-template <int OP>
+// (OP_IWRITE, OP_EXPAND, OP_FILTER). Element e of a chunk goes to thread
+// (e mod 256) in iteration e / 256, and every lane verifies its own element
+// with its own binary searches. With STAGE, an intersection workgroup also
+// keeps one backward adjacency list that is bound to the same vertex across
+// its chunk in LDS and searches it there.
+template <int OP, bool STAGE = false>
 __global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_t* __restrict__ seg_begin, const uint64_t* __restrict__ seg_off,
                                                 uint64_t W, uint64_t n_chunks, uint64_t* __restrict__ mask, uint64_t* __restrict__ chunk_cnt,
                                                 unsigned long long* __restrict__ total, unsigned long long* __restrict__ path_counts) {
-  __shared__ uint64_t s_off[kPathCap + 1];
+  __shared__ int32_t s_rel[kPathCap + 1];
+  __shared__ uint16_t s_pidx[kChunk];
+  __shared__ uint32_t s_scan[kBlock / kWave];
   __shared__ uint64_t s_tmp[kBlock / kWave];
-  __shared__ ChunkPaths s_cp;
+  __shared__ ChunkMap s_cm;
+  __shared__ uint32_t s_list[(STAGE && is_intersect(OP)) ? kStageCap : 1];
+  __shared__ uint32_t s_want[2];
   constexpr bool kCompact = (OP == OP_IWRITE || OP == OP_EXPAND || OP == OP_FILTER);
   const uint64_t P = a.path_num;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   uint64_t acc = 0;
-  for (uint64_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+  // Groups of kGroup consecutive chunks are dealt to the workgroups
+  // round-robin: per-element cost varies strongly along the element range
+  // (hub-heavy paths come first), so round-robin keeps the workgroups
+  // balanced, while a group's consecutive chunks usually continue one run of
+  // paths and reuse the staged list.
+  constexpr uint64_t kGroup = STAGE ? 16 : 1;
+  const uint64_t n_groups = (n_chunks + kGroup - 1) / kGroup;
+  uint32_t stage_v = ~0u, stage_n = 0;  // vertex whose adjacency list s_list holds
+  for (uint64_t gi = blockIdx.x; gi < n_groups; gi += gridDim.x)
+  for (uint64_t chunk = gi * kGroup; chunk < min((gi + 1) * kGroup, n_chunks); ++chunk) {
     const uint64_t e0 = chunk * kChunk;
     const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
-    ChunkPaths cp;
-    stage_chunk(seg_off, P, e0, e1, cp, s_off, &s_cp);
+    ChunkMap cm;
+    build_chunk_map(seg_off, P, e0, e1, cm, s_rel, s_pidx, s_scan, &s_cm);
+    if (STAGE && is_intersect(OP)) {
+      // stage the largest backward list (<= kStageCap entries) that is bound
+      // to the same vertex at both ends of the chunk
+      if (threadIdx.x == 0) {
+        uint32_t best_v = ~0u, best_n = 0;
+        for (uint32_t i = 0; i < a.n_back; ++i) {
+          const uint32_t v = a.inst[a.back[i]][cm.first];
+          if (v != a.inst[a.back[i]][cm.last]) continue;
+          const uint32_t n = (uint32_t)(a.row_ptrs[v + 1] - a.row_ptrs[v]);
+          if (n <= kStageCap && n > best_n) {
+            best_v = v;
+            best_n = n;
+          }
+        }
+        s_want[0] = best_v;
+        s_want[1] = best_n;
+      }
+      __syncthreads();
+      const uint32_t want_v = s_want[0], want_n = s_want[1];
+      if (want_v != ~0u && want_v != stage_v) {
+        const uint64_t b = a.row_ptrs[want_v];
+        for (uint32_t j = threadIdx.x; j < want_n; j += kBlock) s_list[j] = a.cols[b + j];
+        __syncthreads();
+        stage_v = want_v;
+        stage_n = want_n;
+      }
+    }
     uint64_t valid_count = 0;
     for (int it = 0; it < kItems; ++it) {
       const uint64_t e = e0 + (uint64_t)it * kBlock + threadIdx.x;
       bool valid = false;
       if (e < e1) {
-        uint64_t start;
-        uint64_t p = path_of(e, cp, s_off, seg_off, start);
-        uint64_t packed = seg_begin[p];
-        uint64_t tag = packed >> 56;
-        uint32_t c = seg_source<OP>(a, tag)[(packed & kBeginMask) + (e - start)];
-        uint64_t contrib = eval_element<OP>(a, p, tag, c, valid);
+        uint64_t rank;
+        const uint64_t p = chunk_path(e, e0, cm, s_rel, s_pidx, seg_off, rank);
+        const uint64_t packed = seg_begin[p];
+        const uint64_t tag = packed >> 56;
+        const uint32_t c = seg_source<OP>(a, tag)[(packed & kBeginMask) + rank];
+        const uint64_t contrib = eval_element<OP>(a, p, tag, c, valid, stage_v, s_list, stage_n);
         if (OP == OP_IPATH) {
           if (valid) atomicAdd(path_counts + p, 1ull);
         } else if (!kCompact) {
@@ -482,6 +835,103 @@ __global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_
   }
 }
 
+// Map kernel of the intersection and two-set count operators. Each wavefront
+// walks a contiguous run of kItems * 64 elements. An iteration whose 64
+// elements all lie in one path ("uniform", the common case on hub-heavy
+// levels) makes the path wave-uniform, so its bound vertices, row_ptrs and
+// segment come from scalar loads, and verifies the 64 ascending candidates
+// against each other list with the LDS cursor tiles; other iterations verify
+// lane by lane with branchless binary lifting.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_map_isect(vsgm_level_args a, const uint64_t* __restrict__ seg_begin,
+                                                      const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t n_chunks,
+                                                      uint64_t* __restrict__ mask, uint64_t* __restrict__ chunk_cnt,
+                                                      unsigned long long* __restrict__ total, unsigned long long* __restrict__ path_counts) {
+  __shared__ uint64_t s_off[kPathCap + 1];
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  __shared__ uint32_t s_tile[kBlock / kWave][kTile];
+  __shared__ uint32_t s_cursor[kBlock / kWave][kCursorSlots];
+  __shared__ ChunkPaths s_cp;
+  constexpr bool kCompact = (OP == OP_IWRITE);
+  const uint64_t P = a.path_num;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  uint64_t acc = 0;
+  uint64_t cursor_path = ~0ull;  // path whose list positions s_cursor[wave] holds
+  for (uint64_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+    const uint64_t e0 = chunk * kChunk;
+    const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
+    ChunkPaths cp;
+    stage_chunk(seg_off, P, e0, e1, cp, s_off, &s_cp);
+    uint64_t valid_count = 0;
+    const uint64_t wave_base = e0 + (uint64_t)wave * (kItems * kWave);
+    for (int it = 0; it < kItems; ++it) {
+      const uint64_t ef = wave_base + (uint64_t)it * kWave;  // first element of this iteration
+      if (ef >= e1) break;
+      const uint64_t e = ef + lane;
+      const bool active = e < e1;
+      const uint64_t ee = active ? e : ef;  // inactive lanes mirror the first element
+      uint64_t p, start;
+      bool uniform;
+      if (cp.staged) {
+        const uint32_t ns = cp.staged - 1;  // s_off[1..ns] are the path ends
+        const uint32_t top = top_step(ns);
+        const uint32_t i0 = lift_upper_bound(s_off + 1, ns, ef, top);
+        uniform = (ef + kWave <= e1) && (ef + kWave <= s_off[i0 + 1]);
+        const uint32_t i = uniform ? i0 : lift_upper_bound(s_off + 1, ns, ee, top);
+        p = cp.first + i;
+        start = s_off[i];
+      } else {
+        // more paths than fit in LDS (mostly empty segments): global search
+        p = cp.first + last_le_u64(seg_off + cp.first, cp.last - cp.first + 2, ee);
+        start = seg_off[p];
+        uniform = __ballot(!(active && p == __shfl(p, 0))) == 0;
+      }
+      bool valid = false;
+      uint64_t contrib = 0;
+      if (uniform) {
+        p = readfirstlane_u64(p);
+        start = readfirstlane_u64(start);
+        const uint64_t packed = seg_begin[p];
+        const uint32_t tag = (uint32_t)(packed >> 56);
+        const uint32_t c = seg_source<OP>(a, tag)[(packed & kBeginMask) + (e - start)];
+        if (p != cursor_path) {
+          cursor_path = p;
+          if (lane < kCursorSlots) s_cursor[wave][lane] = 0;
+          wave_sync();
+        }
+        contrib = eval_element_wave<OP>(a, p, tag, c, valid, s_tile[wave], s_cursor[wave]);
+      } else {
+        cursor_path = ~0ull;
+        const uint64_t packed = seg_begin[p];
+        const uint32_t tag = (uint32_t)(packed >> 56);
+        const uint32_t c = seg_source<OP>(a, tag)[(packed & kBeginMask) + (ee - start)];
+        contrib = eval_lanes<OP>(a, p, tag, c, active, valid);
+      }
+      valid = valid && active;
+      if (OP == OP_IPATH) {
+        if (valid) atomicAdd(path_counts + p, 1ull);
+      } else if (!kCompact) {
+        acc += active ? contrib : 0;
+      }
+      if (kCompact) {
+        const uint64_t bits = __ballot(valid);
+        if (lane == 0) mask[ef / kWave] = bits;
+        valid_count += valid ? 1 : 0;
+      }
+    }
+    if (kCompact) {
+      uint64_t s = block_sum(valid_count, s_tmp);
+      if (threadIdx.x == 0) chunk_cnt[chunk] = s;
+    }
+    __syncthreads();
+  }
+  if (!kCompact && OP != OP_IPATH) {
+    uint64_t s = block_sum(acc, s_tmp);
+    if (threadIdx.x == 0 && s) atomicAdd(total, (unsigned long long)s);
+  }
+}
+
 // Scatter the valid elements of the compacting ops; chunk_off is the
 // exclusive scan of the chunk counts (n_chunks + 1 entries).
 template <int OP>
@@ -489,10 +939,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uin
                                                     const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t n_chunks,
                                                     const uint64_t* __restrict__ mask, const uint64_t* __restrict__ chunk_off,
                                                     uint32_t* __restrict__ out_vals, uint64_t* __restrict__ out_off, ExpandCols cols) {
-  __shared__ uint64_t s_off[kPathCap + 1];
+  __shared__ int32_t s_rel[kPathCap + 1];
+  __shared__ uint16_t s_pidx[kChunk];
+  __shared__ uint32_t s_scan[kBlock / kWave];
   __shared__ uint64_t s_word[kWordsPerChunk + 1];
   __shared__ uint64_t s_wpre[kWordsPerChunk + 1];
-  __shared__ ChunkPaths s_cp;
+  __shared__ ChunkMap s_cm;
   const uint64_t P = a.path_num;
   const int lane = threadIdx.x & (kWave - 1);
   for (uint64_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
@@ -500,9 +952,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uin
     const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
     const uint32_t n_words = (uint32_t)((e1 - e0 + kWave - 1) / kWave);
     const uint64_t base = chunk_off[chunk];
-    ChunkPaths cp;
+    ChunkMap cm;
     if (threadIdx.x <= kWordsPerChunk) s_word[threadIdx.x] = threadIdx.x < n_words ? mask[e0 / kWave + threadIdx.x] : 0;
-    stage_chunk(seg_off, P, e0, e1, cp, s_off, &s_cp);  // contains the barriers
+    build_chunk_map(seg_off, P, e0, e1, cm, s_rel, s_pidx, s_scan, &s_cm);  // contains the barriers
     if (threadIdx.x == 0) {
       uint64_t run = 0;
       for (uint32_t w = 0; w <= kWordsPerChunk; ++w) {
@@ -518,10 +970,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uin
       const uint64_t word = s_word[w];
       if (!((word >> lane) & 1ull)) continue;
       const uint64_t pos = base + s_wpre[w] + __popcll(word & ((1ull << lane) - 1ull));
-      uint64_t start;
-      uint64_t p = path_of(e, cp, s_off, seg_off, start);
-      uint64_t packed = seg_begin[p];
-      uint32_t c = seg_source<OP>(a, packed >> 56)[(packed & kBeginMask) + (e - start)];
+      uint64_t rank;
+      const uint64_t p = chunk_path(e, e0, cm, s_rel, s_pidx, seg_off, rank);
+      const uint64_t packed = seg_begin[p];
+      const uint32_t c = seg_source<OP>(a, packed >> 56)[(packed & kBeginMask) + rank];
       if (OP == OP_EXPAND) {
         cols.out_inst[a.target][pos] = c;
         for (uint32_t i = 0; i < cols.n_mat; ++i) cols.out_inst[cols.mat[i]][pos] = a.inst[cols.mat[i]][p];
@@ -620,6 +1072,7 @@ struct vsgm_ctx {
   bool profiling = false;
   std::vector<KernelRecord> records;
   std::vector<hipEvent_t> spare_events;
+  vsgm_tuning tuning{0, 8192};
 };
 
 namespace {
@@ -875,6 +1328,30 @@ uint64_t map_bytes(const vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t W, ui
   return b;
 }
 
+// The intersection and two-set count operators use k_map_isect, the
+// candidate-streaming operators (expand, filter) k_map.
+template <int OP>
+void launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t n_chunks,
+                uint64_t* mask, uint64_t* chunk_cnt, unsigned long long* total, unsigned long long* path_counts) {
+  const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(n_chunks, 1), ctx->tuning.max_grid);
+  if constexpr (is_intersect(OP) || OP == OP_COUNT2) {
+    if (ctx->tuning.isect_kernel == 1) {
+      hipLaunchKernelGGL(k_map_isect<OP>, dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask, chunk_cnt, total,
+                         path_counts);
+      return;
+    }
+  }
+  if constexpr (is_intersect(OP)) {
+    if (ctx->tuning.isect_kernel == 2) {
+      hipLaunchKernelGGL((k_map<OP, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask, chunk_cnt,
+                         total, path_counts);
+      return;
+    }
+  }
+  hipLaunchKernelGGL(k_map<OP>, dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask, chunk_cnt, total,
+                     path_counts);
+}
+
 template <int OP>
 int run_count(vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t* d_total, uint64_t* d_path_counts) {
   ctx->stats = {a.path_num, 0, 0, 0};
@@ -887,8 +1364,8 @@ int run_count(vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t* d_total, uint64
   uint64_t n_chunks = (W + kChunk - 1) / kChunk;
   {
     ProfScope ps(ctx, kname<OP>(1), map_bytes<OP>(ctx, a, W, n_chunks), W, a.path_num);
-    hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks,
-                       (uint64_t*)nullptr, (uint64_t*)nullptr, (unsigned long long*)d_total, (unsigned long long*)d_path_counts);
+    launch_map<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, nullptr, nullptr, (unsigned long long*)d_total,
+                   (unsigned long long*)d_path_counts);
   }
   return launch_check(ctx, "k_map(count)");
 }
@@ -918,8 +1395,7 @@ int run_mark(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t W, uint
   VSGM_TRY(t.get(n_chunks + 1, chunk_off));
   {
     ProfScope ps(ctx, kname<OP>(1), map_bytes<OP>(ctx, a, W, n_chunks), W, a.path_num);
-    hipLaunchKernelGGL(k_map<OP>, dim3(grid_for(n_chunks)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, *mask,
-                       *chunk_off, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+    launch_map<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, *mask, *chunk_off, nullptr, nullptr);
   }
   VSGM_TRY(launch_check(ctx, "k_map(mark)"));
   VSGM_TRY(scan_impl(ctx, *chunk_off, *chunk_off, n_chunks));
@@ -1298,6 +1774,19 @@ int vsgm_split_batch(vsgm_ctx* ctx, const uint64_t* d_children, uint64_t n, uint
   *n_batches = ends.size();
   if (!h_ends || ends.size() > max_batches) return fail(ctx, VSGM_ERR_ARG, "split needs %llu batch slots", (unsigned long long)ends.size());
   std::copy(ends.begin(), ends.end(), h_ends);
+  return VSGM_OK;
+}
+
+int vsgm_set_tuning(vsgm_ctx* ctx, const vsgm_tuning* t) {
+  if (!ctx || !t) return VSGM_ERR_ARG;
+  if (t->isect_kernel > 2 || t->max_grid == 0 || t->max_grid > (1u << 20)) return fail(ctx, VSGM_ERR_ARG, "bad tuning");
+  ctx->tuning = *t;
+  return VSGM_OK;
+}
+
+int vsgm_get_tuning(const vsgm_ctx* ctx, vsgm_tuning* t) {
+  if (!ctx || !t) return VSGM_ERR_ARG;
+  *t = ctx->tuning;
   return VSGM_OK;
 }
 

@@ -313,7 +313,13 @@ int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
     if (k == "batch_bytes") o.batch_bytes = (uint64_t)value;
     else if (k == "min_batch_bytes") o.min_batch_bytes = (uint64_t)value;
     else if (k == "profile") o.profile = value != 0;
-    else throw vsgm::Error("unknown executor option " + k);
+    else if (k == "isect_kernel" || k == "max_grid") {
+      vsgm_tuning t;
+      vsgm_get_tuning(e->e->Context(), &t);
+      if (k == "isect_kernel") t.isect_kernel = (uint32_t)value;
+      else t.max_grid = (uint32_t)value;
+      if (vsgm_set_tuning(e->e->Context(), &t) != VSGM_OK) throw vsgm::Error("bad tuning value for " + k);
+    } else throw vsgm::Error("unknown executor option " + k);
   });
 }
 
