@@ -94,7 +94,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--detail-out", default="")
+    ap.add_argument("--hub-bytes", type=float, default=-1, help="membership index bytes (-1 auto, 0 off)")
+    ap.add_argument("--hub-min-degree", type=int, default=64)
+    ap.add_argument("--isect-kernel", type=int, default=0)
+    ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
     args = ap.parse_args()
+    os.environ["VSGM_PROGRESS"] = str(args.progress)
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by view bins "
@@ -128,10 +133,14 @@ def main():
     log(f"view: hop={hop} root_degree={root_degree} vertices={info['vertices']} entries={info['entries']} in {view_s:.1f}s")
 
     ex = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
+    ex.set_option("hub_bytes", args.hub_bytes)
+    ex.set_option("hub_min_degree", args.hub_min_degree)
+    ex.set_option("isect_kernel", args.isect_kernel)
     t = time.perf_counter()
     ex.transfer(holder)
     xfer_s = time.perf_counter() - t
-    log(f"transfer {info['entries'] * 4 / 1e9 + (g.vertex_count + 1) * 8 / 1e9:.2f} GB in {xfer_s:.2f}s")
+    log(f"transfer {info['entries'] * 4 / 1e9 + (g.vertex_count + 1) * 8 / 1e9:.2f} GB + membership index of "
+        f"{ex.hub_count()} lists in {xfer_s:.2f}s")
 
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -213,6 +222,8 @@ def main():
             "graph_gen_s": round(gen_s, 2),
             "view_build_s": round(view_s, 2),
             "transfer_s": round(xfer_s, 3),
+            "index_lists": ex.hub_count(),
+            "isect_kernel": args.isect_kernel,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -106,9 +106,14 @@ typedef struct vsgm_level_stat {
 int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* const* plans, int n_plans, uint64_t max_sources,
                          uint64_t mem_budget, vsgm_executor** out);
 /* Options: "batch_bytes" (0 = budget / remaining levels), "min_batch_bytes",
- * "profile" (1 = time every level op), and the device tuning knobs
- * "isect_kernel" and "max_grid" (vsgm_tuning in vsgm_hip.h). */
+ * "profile" (1 = time every level op), "hub_bytes" (device bytes for the
+ * dense membership index of the highest-degree view lists, built at the next
+ * transfer; 0 = off, -1 = min(budget/8, 16 GiB), the default),
+ * "hub_min_degree" (smallest indexed degree, default 64), and the device
+ * tuning knobs "isect_kernel" and "max_grid" (vsgm_tuning in vsgm_hip.h). */
 int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value);
+/* Number of view vertices in the current membership index. */
+int vsgm_executor_hub_count(const vsgm_executor* e, uint32_t* n);
 int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h);
 /* Whole graph as one view; sources NULL = every vertex. */
 int vsgm_executor_transfer_graph(vsgm_executor* e, const vsgm_graph* g, const uint32_t* sources, uint64_t n_sources);

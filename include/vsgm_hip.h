@@ -104,7 +104,11 @@ int vsgm_fill_u64(vsgm_ctx* ctx, uint64_t* d, uint64_t n, uint64_t value);
  *   cond[]   conditions "candidate OP inst[operand]" (computed / materialized
  *            / filter ordering of the target, depending on the op);
  *   other    second computed, unmaterialized vertex counted against the
- *            target's candidates (-1: none), with "target other_op other". */
+ *            target's candidates (-1: none), with "target other_op other".
+ *   hub_*    optional dense membership index of the same view (see
+ *            vsgm_hub_index_build; hub_slot NULL: none). Membership tests
+ *            against an indexed list read one bit instead of a binary search;
+ *            results are identical either way. */
 typedef struct vsgm_cond {
   uint32_t op;
   uint32_t operand;
@@ -127,7 +131,28 @@ typedef struct vsgm_level_args {
   vsgm_cond cond[VSGM_MAX_QV];
   int32_t other;
   uint32_t other_op;
+  const uint32_t* hub_slot;  /* u32[V]: bitmap row of an indexed vertex, ~0u otherwise */
+  const uint32_t* hub_bits;  /* u32[n_hubs * hub_words]: bit c of row s = c in adj(hub s) */
+  uint64_t hub_words;        /* u32 words per bitmap row */
 } vsgm_level_args;
+
+/* Dense membership index of the highest-degree lists of a view. No
+ * counterpart in the reference, whose connectivity check always binary-
+ * searches the list (ThreadCheckConnectivity / BinSearch,
+ * src/gpu/gpu_check_constraints_common.h:11-63, gpu_utils_bin_search.h:17-29);
+ * it trades HBM (V/8 bytes per indexed vertex) for one probe per test. */
+typedef struct vsgm_hub_index {
+  uint32_t* slot;      /* u32[vertex_count] */
+  uint32_t* bits;      /* u32[n_hubs * words] */
+  uint64_t words;
+  uint32_t n_hubs;
+} vsgm_hub_index;
+/* Build the index of the n_hubs vertices h_hubs[] (host array, distinct ids
+ * < vertex_count) over the device view (d_row_ptrs, d_cols); memory comes
+ * from the context pool. */
+int vsgm_hub_index_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* h_hubs,
+                         uint32_t n_hubs, vsgm_hub_index* out);
+int vsgm_hub_index_free(vsgm_ctx* ctx, vsgm_hub_index* index);
 
 /* A candidate CSR produced by an op: offsets u64[path_num+1] and values
  * u32[count], both pool-allocated (free with vsgm_free). */

@@ -17,10 +17,12 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def run_gpu(vsgm, graph, plans, sources=None, batch_bytes=None):
+def run_gpu(vsgm, graph, plans, sources=None, batch_bytes=None, **options):
     ex = vsgm.PipelineExecutor(0, graph.vertex_count, plans)
     if batch_bytes:
         ex.set_option("batch_bytes", batch_bytes)
+    for k, v in options.items():
+        ex.set_option(k, v)
     ex.transfer_graph(graph, sources)
     ex.match()
     return ex.total_counts
@@ -85,6 +87,23 @@ def test_closed_forms(vsgm, oracle, gpu):
     # wheel W_12 (hub + 12-cycle): 12 triangles
     wheel = vsgm.Graph.from_edges(13, [(0, i) for i in range(1, 13)] + [(i, i % 12 + 1) for i in range(1, 13)])
     assert run_gpu(vsgm, wheel, [vsgm.Plan.optimize(vsgm.Query.clique(3))]) == [12]
+
+
+def test_hub_index_invariance(vsgm, oracle, rmat9):
+    """Counts are the same with the membership index off, on for the
+    largest lists only, and on for every non-empty list."""
+    plans = [vsgm.Plan.optimize(q) for q in [vsgm.Query.preset(k) for k in range(19)] + [vsgm.Query.clique(5)]]
+    want = oracle_counts(oracle, rmat9, plans)
+    for opts in ({"hub_bytes": 0}, {"hub_min_degree": 32}, {"hub_min_degree": 1}, {"hub_min_degree": 1, "hub_bytes": 64 * 40}):
+        print(f"[gpu] hub index {opts}", flush=True)
+        assert run_gpu(vsgm, rmat9, plans, **opts) == want, opts
+    ex = vsgm.PipelineExecutor(0, rmat9.vertex_count, plans[:1])
+    ex.set_option("hub_min_degree", 1)
+    ex.transfer_graph(rmat9)
+    assert ex.hub_count() == int(np.count_nonzero(np.diff(rmat9.csr()[0])))
+    ex.set_option("hub_bytes", 64 * 40)  # 40 rows of 512 bits
+    ex.transfer_graph(rmat9)
+    assert ex.hub_count() == 40
 
 
 def test_batching_invariance(vsgm, oracle, rmat9):

@@ -183,6 +183,82 @@ def test_intersect_write_expand_filter_count(ctx):
     assert int(tot.numpy()[0]) == sum(len(r) * (len(r) - 1) // 2 for r in want_rows)
 
 
+def test_hub_index_bits_and_indexed_intersections(ctx):
+    """The dense membership index holds exactly each indexed list, and
+    intersections that probe it give the same rows as binary search."""
+    import ctypes as C
+
+    from vsgm_amd.hip import CsrOut, DeviceArray, hiplib
+
+    rng = np.random.default_rng(21)
+    n = 5000
+    edges = rng.integers(0, n, size=(60_000, 2))
+    edges[:6000, 0] = rng.integers(0, 30, size=6000)
+    rp, cols, adj = _csr_from_edges(n, edges)
+    sets = [set(x) for x in adj]
+    deg = np.diff(rp).astype(np.int64)
+    hubs = np.nonzero(deg >= 40)[0].astype(np.uint32)
+    rng.shuffle(hubs)  # slot order is the caller's order
+    assert 20 < hubs.size < n // 4
+    d_rp, d_cols = ctx.array(rp), ctx.array(cols)
+    idx = ctx.hub_index(d_rp, d_cols, n, hubs)
+    try:
+        assert idx.n_hubs == hubs.size and idx.words % 2 == 0 and idx.words * 32 >= n
+        slot = DeviceArray(ctx, idx.slot, n, np.uint32, owned=False).numpy()
+        want_slot = np.full(n, 0xFFFFFFFF, np.uint32)
+        want_slot[hubs] = np.arange(hubs.size, dtype=np.uint32)
+        assert np.array_equal(slot, want_slot)
+        bits = DeviceArray(ctx, idx.bits, hubs.size * idx.words, np.uint32, owned=False).numpy().reshape(hubs.size, idx.words)
+        for s, v in enumerate(hubs.tolist()):
+            members = np.nonzero(np.unpackbits(bits[s].view(np.uint8), bitorder="little"))[0]
+            assert members.tolist() == adj[v], v
+
+        # wedge paths (v0, v1, v2) with v1 < v2 in N(v0); u3 in N(v0) ∩ N(v1) ∩ N(v2), u3 > v0
+        v0s, v1s, v2s = [], [], []
+        for v in range(0, n, 7):
+            nb = adj[v]
+            for i in range(min(len(nb), 12)):
+                for j in range(i + 1, min(len(nb), 12)):
+                    v0s.append(v), v1s.append(nb[i]), v2s.append(nb[j])
+        P = len(v0s)
+        cols_d = [ctx.array(np.array(x, np.uint32)) for x in (v0s, v1s, v2s)]
+        rows = [sorted(x for x in sets[a] & sets[b] & sets[c] if x > a) for a, b, c in zip(v0s, v1s, v2s)]
+        want_off = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint64)
+        assert want_off[-1] > 0
+        for use_index in (False, True):
+            a = _args(ctx, d_rp, d_cols, n, P)
+            for u in range(3):
+                a.inst[u] = cols_d[u].ptr
+            a.target, a.n_back = 3, 3
+            a.back[0], a.back[1], a.back[2] = 0, 1, 2
+            a.n_cond = 1
+            a.cond[0].op, a.cond[0].operand = 1, 0  # > v0
+            if use_index:
+                a.hub_slot, a.hub_bits, a.hub_words = idx.slot, idx.bits, idx.words
+            out = CsrOut()
+            ctx.check(hiplib().vsgm_intersect_write(ctx.h, C.byref(a), C.byref(out)), "intersect_write")
+            assert np.array_equal(ctx.adopt(out.offsets, P + 1, np.uint64).numpy(), want_off)
+            assert np.array_equal(ctx.adopt(out.values, out.count, np.uint32).numpy(), np.array([x for r in rows for x in r], np.uint32))
+            tot = ctx.array(np.zeros(1, np.uint64))
+            ctx.check(hiplib().vsgm_intersect_count_total(ctx.h, C.byref(a), tot.ptr), "count_total")
+            assert int(tot.numpy()[0]) == int(want_off[-1])
+    finally:
+        ctx.hub_index_free(idx)
+
+
+def test_hub_index_rejects_bad_lists(ctx):
+    from vsgm_amd.hip import VsgmError
+
+    rp, cols, _ = _csr_from_edges(4, [(0, 1), (1, 2)])
+    d_rp, d_cols = ctx.array(rp), ctx.array(cols)
+    for bad in ([1, 1], [4]):
+        with pytest.raises(VsgmError):
+            ctx.hub_index(d_rp, d_cols, 4, np.array(bad, np.uint32))
+    empty = ctx.hub_index(d_rp, d_cols, 4, np.zeros(0, np.uint32))
+    assert empty.n_hubs == 0
+    ctx.hub_index_free(empty)
+
+
 def test_level_args_are_validated(ctx):
     import ctypes as C
 

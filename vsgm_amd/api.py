@@ -125,6 +125,7 @@ def _declare(L):
         "vsgm_executor_reset": ([P], C.c_int),
         "vsgm_executor_level_stats": ([P, C.c_int, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_peak_bytes": ([P, C.POINTER(u64)], C.c_int),
+        "vsgm_executor_hub_count": ([P, C.POINTER(C.c_uint32)], C.c_int),
         "vsgm_executor_context": ([P, PP], C.c_int),
         "vsgm_profile_enable": ([P, C.c_int], C.c_int),
         "vsgm_profile_collect": ([P, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
@@ -493,6 +494,12 @@ class PipelineExecutor(_Handle):
         b = u64()
         _check(lib().vsgm_executor_peak_bytes(self._h, C.byref(b)), "peak")
         return b.value
+
+    def hub_count(self) -> int:
+        """Vertices of the current view in the dense membership index."""
+        n = C.c_uint32()
+        _check(lib().vsgm_executor_hub_count(self._h, C.byref(n)), "hub count")
+        return n.value
 
     def _ctx(self):
         c = C.c_void_p()

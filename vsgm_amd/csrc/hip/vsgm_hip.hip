@@ -25,8 +25,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -42,6 +44,7 @@ constexpr int kItems = 8;                 // elements per thread per chunk
 constexpr int kChunk = kBlock * kItems;   // elements per chunk (multiple of 64)
 constexpr int kWordsPerChunk = kChunk / kWave;
 constexpr int kPathCap = kChunk;          // paths whose offsets fit in LDS per chunk
+constexpr int kPhaseItems = 4;            // elements a thread verifies in one phased group
 constexpr uint64_t kBeginMask = (1ull << 56) - 1;
 constexpr int kMaxGrid = 8192;
 
@@ -389,6 +392,27 @@ __global__ __launch_bounds__(kBlock) void k_fill(uint64_t* d, uint64_t n, uint64
   for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (uint64_t)gridDim.x * kBlock) d[e] = v;
 }
 
+// Hub index: slot[hubs[i]] = i (slot pre-filled with ~0u), then one
+// workgroup per hub sets the bits of its adjacency list. Lists are sorted,
+// so neighbouring lanes often hit the same word: atomicOr merges them.
+__global__ __launch_bounds__(kBlock) void k_hub_slots(const uint32_t* __restrict__ hubs, uint32_t n_hubs, uint32_t* __restrict__ slot) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_hubs; i += gridDim.x * kBlock) slot[hubs[i]] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hub_bits(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ cols,
+                                                     uint64_t vertex_count, const uint32_t* __restrict__ hubs, uint32_t n_hubs,
+                                                     uint32_t* __restrict__ bits, uint64_t words) {
+  for (uint32_t s = blockIdx.x; s < n_hubs; s += gridDim.x) {
+    const uint32_t v = hubs[s];
+    const uint64_t b = row_ptrs[v], e = row_ptrs[v + 1];
+    uint32_t* row = bits + (uint64_t)s * words;
+    for (uint64_t j = b + threadIdx.x; j < e; j += kBlock) {
+      const uint32_t c = cols[j];
+      if (c < vertex_count) atomicOr(row + (c >> 5), 1u << (c & 31));
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // level operators
 
@@ -476,6 +500,14 @@ __device__ inline uint64_t eval_element(const vsgm_level_args& a, uint64_t p, ui
     for (uint32_t i = 0; i < a.n_back && ok; ++i) {
       if (i == tag) continue;
       uint32_t v = a.inst[a.back[i]][p];
+      if (a.hub_slot) {
+        // indexed list: one bit probe instead of a binary search
+        const uint32_t s = a.hub_slot[v];
+        if (s != ~0u) {
+          ok = (a.hub_bits[(uint64_t)s * a.hub_words + (c >> 5)] >> (c & 31)) & 1u;
+          continue;
+        }
+      }
       if (v == stage_v) {
         // the workgroup holds this adjacency list in LDS
         uint32_t pos = lower_bound_u32(s_list, stage_n, c);
@@ -660,74 +692,77 @@ __device__ inline uint64_t path_of(uint64_t e, const ChunkPaths& cp, const uint6
   return cp.first + i;
 }
 
-// Element -> path map of one chunk, built once per chunk in LDS: every path
-// whose non-empty segment starts inside the chunk writes its index at its
-// start position, and an inclusive max-scan over the chunk fills the gaps,
-// so each element finds its path with a single LDS read (load-balance
-// search by scatter + scan). s_rel[i] = segment start of path first+i
-// relative to the chunk start (negative for the path that began earlier).
+// Element -> path map of one chunk: the segment starts of the chunk's paths,
+// relative to the chunk start, are staged in LDS once per chunk
+// (s_rel[i] = start of path first+i, negative for the path that began
+// earlier), and each element finds its path with a binary search there
+// (load-balanced search over an LDS-resident offset window). A chunk that
+// touches more than kPathCap paths searches the global offsets instead.
 struct ChunkMap {
   uint64_t first;    // path of the chunk's first element
   uint64_t last;     // path of the chunk's last element
-  uint32_t mapped;   // 1: s_pidx / s_rel valid; 0: more paths than kPathCap
+  uint32_t mapped;   // 1: s_rel valid; 0: more paths than kPathCap
 };
 
-__device__ inline void build_chunk_map(const uint64_t* __restrict__ seg_off, uint64_t P, uint64_t e0, uint64_t e1, ChunkMap& cm,
-                                       int32_t* s_rel, uint16_t* s_pidx, uint32_t* s_scan, ChunkMap* s_cm) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  if (threadIdx.x == 0) {
-    ChunkMap c;
-    c.first = last_le_u64(seg_off, P + 1, e0);
-    c.last = last_le_u64(seg_off, P + 1, e1 - 1);
-    c.mapped = (c.last - c.first + 1) <= (uint64_t)kPathCap ? 1u : 0u;
-    *s_cm = c;
-  }
-  __syncthreads();
-  cm = *s_cm;
-  if (!cm.mapped) return;
-  const uint32_t n_paths = (uint32_t)(cm.last - cm.first + 1);
-  const uint32_t len = (uint32_t)(e1 - e0);
-  for (uint32_t i = threadIdx.x; i <= n_paths; i += kBlock) s_rel[i] = (int32_t)((int64_t)seg_off[cm.first + i] - (int64_t)e0);
-  for (uint32_t j = threadIdx.x; j < (uint32_t)kChunk; j += kBlock) s_pidx[j] = 0;
-  __syncthreads();
-  // paths 1..n_paths-1 start inside the chunk (s_rel >= 1); empty ones write nothing
-  for (uint32_t i = 1 + threadIdx.x; i < n_paths; i += kBlock) {
-    const int32_t s = s_rel[i];
-    if (s_rel[i + 1] > s) s_pidx[s] = (uint16_t)i;
-  }
-  __syncthreads();
-  // inclusive max-scan; thread t owns entries [8t, 8t + 8)
-  uint32_t loc[kItems];
-  uint32_t m = 0;
-  for (int k = 0; k < kItems; ++k) {
-    const uint32_t idx = threadIdx.x * kItems + k;
-    m = max(m, idx < len ? (uint32_t)s_pidx[idx] : 0u);
-    loc[k] = m;
-  }
-  uint32_t x = m;
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d);
-    if (lane >= d) x = max(x, y);
-  }
-  if (lane == kWave - 1) s_scan[wave] = x;
-  __syncthreads();
-  uint32_t before = __shfl_up(x, 1);
-  before = lane ? before : 0u;
-  for (int w = 0; w < wave; ++w) before = max(before, s_scan[w]);
-  for (int k = 0; k < kItems; ++k) {
-    const uint32_t idx = threadIdx.x * kItems + k;
-    if (idx < len) s_pidx[idx] = (uint16_t)max(before, loc[k]);
+// The chunks one workgroup processes, in increasing order: groups of kGroup
+// consecutive chunks (counted from c_lo) dealt round-robin to the workgroups.
+template <uint64_t kGroup>
+__device__ inline uint64_t block_chunk(uint64_t k, uint64_t c_lo = 0) {
+  return c_lo + ((uint64_t)blockIdx.x + (k / kGroup) * gridDim.x) * kGroup + k % kGroup;
+}
+
+// First and last path (and, for the CSR-writing scatter, the range of paths
+// whose segment starts in the chunk) of the workgroup's next kBlock chunks,
+// one chunk per thread: the binary searches over the path offsets run in
+// parallel instead of one thread's serial searches per chunk.
+struct ChunkTable {
+  uint64_t first[kBlock];
+  uint64_t last[kBlock];
+  uint64_t p_lo[kBlock];
+  uint64_t p_hi[kBlock];
+};
+
+template <uint64_t kGroup, bool kStarts>
+__device__ inline void fill_chunk_table(const uint64_t* __restrict__ seg_off, uint64_t P, uint64_t W, uint64_t n_chunks, uint64_t k0,
+                                        ChunkTable* tab, uint64_t c_lo = 0, uint64_t c_hi = ~0ull) {
+  const uint64_t chunk = block_chunk<kGroup>(k0 + threadIdx.x, c_lo);
+  if (chunk < n_chunks && chunk < c_hi) {
+    const uint64_t e0 = chunk * kChunk;
+    const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
+    tab->first[threadIdx.x] = last_le_u64(seg_off, P + 1, e0);
+    tab->last[threadIdx.x] = last_le_u64(seg_off, P + 1, e1 - 1);
+    if (kStarts) {
+      tab->p_lo[threadIdx.x] = first_ge_u64(seg_off, P + 1, e0);
+      tab->p_hi[threadIdx.x] = chunk == n_chunks - 1 ? P + 1 : first_ge_u64(seg_off, P + 1, e1);
+    }
   }
   __syncthreads();
 }
 
+__device__ inline void build_chunk_map(const uint64_t* __restrict__ seg_off, uint64_t first, uint64_t last, uint64_t e0, ChunkMap& cm,
+                                       int32_t* s_rel) {
+  cm.first = first;
+  cm.last = last;
+  cm.mapped = (last - first + 1) <= (uint64_t)kPathCap ? 1u : 0u;
+  if (!cm.mapped) return;
+  const uint32_t n_paths = (uint32_t)(cm.last - cm.first + 1);
+  for (uint32_t i = threadIdx.x; i <= n_paths; i += kBlock) s_rel[i] = (int32_t)((int64_t)seg_off[cm.first + i] - (int64_t)e0);
+  __syncthreads();
+}
+
 // Path of element e in [e0, e1) and the element's rank inside its segment.
-__device__ inline uint64_t chunk_path(uint64_t e, uint64_t e0, const ChunkMap& cm, const int32_t* s_rel, const uint16_t* s_pidx,
-                                      const uint64_t* __restrict__ seg_off, uint64_t& rank) {
+__device__ inline uint64_t chunk_path(uint64_t e, uint64_t e0, const ChunkMap& cm, const int32_t* s_rel, const uint64_t* __restrict__ seg_off,
+                                      uint64_t& rank) {
   if (cm.mapped) {
-    const uint32_t j = (uint32_t)(e - e0);
-    const uint32_t i = s_pidx[j];
+    // last staged path whose segment starts at or before the element
+    const int32_t j = (int32_t)(e - e0);
+    uint32_t lo = 0, hi = (uint32_t)(cm.last - cm.first + 2);
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_rel[mid] <= j) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t i = lo - 1;
     rank = (uint64_t)((int64_t)j - (int64_t)s_rel[i]);
     return cm.first + i;
   }
@@ -736,21 +771,202 @@ __device__ inline uint64_t chunk_path(uint64_t e, uint64_t e0, const ChunkMap& c
   return cm.first + i;
 }
 
+// Intersection verification of a lane's elements base + it * 64 + lane of
+// the wavefront's run (it in [it0, it0 + kN)), phase by phase: every phase
+// issues its kN independent loads back to back before any result is used,
+// so one wavefront keeps up to kN dependent-load chains in flight (path ->
+// segment -> candidate -> bound vertex -> index row -> bit) instead of one.
+// Sets valid[it]; returns the summed count contribution (OP_ICOUNT) and
+// bumps path_counts (OP_IPATH).
+template <int OP, int kN>
+__device__ inline uint64_t verify_items(const vsgm_level_args& a, const uint64_t* __restrict__ seg_begin, const uint64_t* __restrict__ seg_off,
+                                        uint64_t e0, uint64_t base, uint64_t e1, const ChunkMap& cm, const int32_t* s_rel, int it0, bool* valid,
+                                        unsigned long long* __restrict__ path_counts) {
+  uint64_t p[kN];
+  uint32_t c[kN], pivot[kN];
+  bool ok[kN];
+  {
+    uint32_t rank[kN];
+#pragma unroll
+    for (int it = 0; it < kN; ++it) {
+      const uint64_t e = base + (uint64_t)(it0 + it) * kWave + (threadIdx.x & (kWave - 1));
+      ok[it] = e < e1;
+      uint64_t r = 0;
+      p[it] = ok[it] ? chunk_path(e, e0, cm, s_rel, seg_off, r) : cm.first;
+      rank[it] = (uint32_t)r;
+    }
+    uint64_t packed[kN];
+#pragma unroll
+    for (int it = 0; it < kN; ++it) packed[it] = ok[it] ? seg_begin[p[it]] : 0;
+#pragma unroll
+    for (int it = 0; it < kN; ++it) {
+      pivot[it] = (uint32_t)(packed[it] >> 56);
+      c[it] = ok[it] ? a.cols[(packed[it] & kBeginMask) + rank[it]] : 0u;
+    }
+  }
+  for (uint32_t i = 0; i < a.n_back; ++i) {
+    const uint32_t* col = a.inst[a.back[i]];
+    uint32_t v[kN], s[kN], w[kN];
+#pragma unroll
+    for (int it = 0; it < kN; ++it) v[it] = (ok[it] && pivot[it] != i) ? col[p[it]] : ~0u;
+    if (a.hub_slot) {
+#pragma unroll
+      for (int it = 0; it < kN; ++it) s[it] = v[it] != ~0u ? a.hub_slot[v[it]] : ~0u;
+#pragma unroll
+      for (int it = 0; it < kN; ++it) w[it] = s[it] != ~0u ? a.hub_bits[(uint64_t)s[it] * a.hub_words + (c[it] >> 5)] : 0u;
+    } else {
+#pragma unroll
+      for (int it = 0; it < kN; ++it) s[it] = ~0u, w[it] = 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < kN; ++it) {
+      if (v[it] == ~0u) continue;
+      if (s[it] != ~0u) {
+        ok[it] = (w[it] >> (c[it] & 31)) & 1u;
+      } else {
+        const uint64_t b = a.row_ptrs[v[it]];
+        const uint32_t n = (uint32_t)(a.row_ptrs[v[it] + 1] - b);
+        const uint32_t pos = lower_bound_u32(a.cols + b, n, c[it]);
+        ok[it] = pos < n && a.cols[b + pos] == c[it];
+      }
+    }
+  }
+  for (uint32_t i = 0; i < a.n_cond; ++i) {
+    if (a.cond[i].op != VSGM_NON_EQUAL) continue;
+    const uint32_t* col = a.inst[a.cond[i].operand];
+#pragma unroll
+    for (int it = 0; it < kN; ++it)
+      if (ok[it] && col[p[it]] == c[it]) ok[it] = false;
+  }
+  uint64_t acc = 0;
+#pragma unroll
+  for (int it = 0; it < kN; ++it) {
+    valid[it0 + it] = ok[it];
+    if (!ok[it]) continue;
+    if (OP == OP_IPATH) {
+      atomicAdd(path_counts + p[it], 1ull);
+    } else if (OP == OP_ICOUNT) {
+      if (a.other >= 0) {
+        const uint64_t r = cand_row(a, (uint32_t)a.other, p[it]);
+        const uint64_t b = a.cand_off[a.other][r];
+        const uint32_t n = (uint32_t)(a.cand_off[a.other][r + 1] - b);
+        acc += count_against(a.cand[a.other] + b, n, c[it], a.other_op);
+      } else {
+        acc += 1;
+      }
+    }
+  }
+  return acc;
+}
+
+// Intersection verification of one wavefront's run of kItems * 64
+// consecutive elements [base, min(base + 512, e1)), element base + it * 64 +
+// lane per lane. When the whole run lies in one path (the common case on
+// hub-heavy levels, whose segments are long) the path's segment, bound
+// vertices, index rows and condition operands are wave-uniform and read once
+// with scalar loads; each lane then only streams its candidates (coalesced)
+// and probes the index bits or searches the lists. Other runs take the
+// per-lane phased path. Sets valid[it] for every item.
+template <int OP>
+__device__ inline uint64_t verify_run(const vsgm_level_args& a, const uint64_t* __restrict__ seg_begin, const uint64_t* __restrict__ seg_off,
+                                      uint64_t e0, uint64_t base, uint64_t e1, const ChunkMap& cm, const int32_t* s_rel, bool (&valid)[kItems],
+                                      unsigned long long* __restrict__ path_counts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (base >= e1) {
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) valid[it] = false;
+    return 0;
+  }
+  const uint64_t last = min(base + (uint64_t)(kItems * kWave), e1) - 1;
+  uint64_t r_first = 0, r_last = 0;
+  const uint64_t p = readfirstlane_u64(chunk_path(base, e0, cm, s_rel, seg_off, r_first));
+  const uint64_t p_last = readfirstlane_u64(chunk_path(last, e0, cm, s_rel, seg_off, r_last));
+  if (p != p_last) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int it0 = 0; it0 < kItems; it0 += kPhaseItems)
+      acc += verify_items<OP, kPhaseItems>(a, seg_begin, seg_off, e0, base, e1, cm, s_rel, it0, valid, path_counts);
+    return acc;
+  }
+  // one path: candidate of element e is cols[begin + (e - start)]
+  const uint64_t packed = seg_begin[p];
+  const uint32_t pivot = (uint32_t)(packed >> 56);
+  const uint64_t start = base - readfirstlane_u64(r_first);
+  const uint64_t begin = packed & kBeginMask;
+  uint32_t c[kItems];
+  bool ok[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t e = base + (uint64_t)it * kWave + lane;
+    ok[it] = e <= last;
+    c[it] = ok[it] ? a.cols[begin + (e - start)] : 0u;
+  }
+  for (uint32_t i = 0; i < a.n_back; ++i) {
+    if (i == pivot) continue;
+    const uint32_t v = a.inst[a.back[i]][p];
+    const uint32_t s = a.hub_slot ? a.hub_slot[v] : ~0u;
+    if (s != ~0u) {
+      const uint32_t* row = a.hub_bits + (uint64_t)s * a.hub_words;
+      uint32_t w[kItems];
+#pragma unroll
+      for (int it = 0; it < kItems; ++it) w[it] = ok[it] ? row[c[it] >> 5] : 0u;
+#pragma unroll
+      for (int it = 0; it < kItems; ++it) ok[it] = ok[it] && ((w[it] >> (c[it] & 31)) & 1u);
+    } else {
+      const uint64_t b = a.row_ptrs[v];
+      const uint32_t n = (uint32_t)(a.row_ptrs[v + 1] - b);
+#pragma unroll
+      for (int it = 0; it < kItems; ++it) {
+        if (!ok[it]) continue;
+        const uint32_t pos = lower_bound_u32(a.cols + b, n, c[it]);
+        ok[it] = pos < n && a.cols[b + pos] == c[it];
+      }
+    }
+  }
+  for (uint32_t i = 0; i < a.n_cond; ++i) {
+    if (a.cond[i].op != VSGM_NON_EQUAL) continue;
+    const uint32_t m = a.inst[a.cond[i].operand][p];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) ok[it] = ok[it] && c[it] != m;
+  }
+  uint64_t acc = 0;
+  if (OP == OP_ICOUNT && a.other >= 0) {
+    const uint64_t r = cand_row(a, (uint32_t)a.other, p);
+    const uint64_t b = a.cand_off[a.other][r];
+    const uint32_t n = (uint32_t)(a.cand_off[a.other][r + 1] - b);
+#pragma unroll
+    for (int it = 0; it < kItems; ++it)
+      if (ok[it]) acc += count_against(a.cand[a.other] + b, n, c[it], a.other_op);
+  } else {
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) acc += ok[it] ? 1 : 0;
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) valid[it] = ok[it];
+  if (OP == OP_IPATH) {
+    // one atomic per wavefront: every valid element belongs to path p
+    const uint64_t n_valid = wave_sum(acc);
+    if (lane == 0 && n_valid) atomicAdd(path_counts + p, (unsigned long long)n_valid);
+    return 0;
+  }
+  return acc;
+}
+
 // Map: count (OP_ICOUNT, OP_COUNT2, OP_IPATH) or mark valid elements
 // (OP_IWRITE, OP_EXPAND, OP_FILTER). Element e of a chunk goes to thread
 // (e mod 256) in iteration e / 256, and every lane verifies its own element
 // with its own binary searches. With STAGE, an intersection workgroup also
 // keeps one backward adjacency list that is bound to the same vertex across
-// its chunk in LDS and searches it there.
+// its chunk in LDS and searches it there. One launch covers the chunk
+// window [c_lo, c_hi) of the n_chunks chunks.
 template <int OP, bool STAGE = false>
 __global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_t* __restrict__ seg_begin, const uint64_t* __restrict__ seg_off,
-                                                uint64_t W, uint64_t n_chunks, uint64_t* __restrict__ mask, uint64_t* __restrict__ chunk_cnt,
-                                                unsigned long long* __restrict__ total, unsigned long long* __restrict__ path_counts) {
+                                                uint64_t W, uint64_t n_chunks, uint64_t c_lo, uint64_t c_hi, uint64_t* __restrict__ mask,
+                                                uint64_t* __restrict__ chunk_cnt, unsigned long long* __restrict__ total,
+                                                unsigned long long* __restrict__ path_counts) {
   __shared__ int32_t s_rel[kPathCap + 1];
-  __shared__ uint16_t s_pidx[kChunk];
-  __shared__ uint32_t s_scan[kBlock / kWave];
   __shared__ uint64_t s_tmp[kBlock / kWave];
-  __shared__ ChunkMap s_cm;
+  __shared__ ChunkTable s_tab;
   __shared__ uint32_t s_list[(STAGE && is_intersect(OP)) ? kStageCap : 1];
   __shared__ uint32_t s_want[2];
   constexpr bool kCompact = (OP == OP_IWRITE || OP == OP_EXPAND || OP == OP_FILTER);
@@ -764,14 +980,16 @@ __global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_
   // balanced, while a group's consecutive chunks usually continue one run of
   // paths and reuse the staged list.
   constexpr uint64_t kGroup = STAGE ? 16 : 1;
-  const uint64_t n_groups = (n_chunks + kGroup - 1) / kGroup;
   uint32_t stage_v = ~0u, stage_n = 0;  // vertex whose adjacency list s_list holds
-  for (uint64_t gi = blockIdx.x; gi < n_groups; gi += gridDim.x)
-  for (uint64_t chunk = gi * kGroup; chunk < min((gi + 1) * kGroup, n_chunks); ++chunk) {
+  for (uint64_t k0 = 0; block_chunk<kGroup>(k0, c_lo) < c_hi; k0 += kBlock) {
+  fill_chunk_table<kGroup, false>(seg_off, P, W, n_chunks, k0, &s_tab, c_lo, c_hi);
+  for (uint32_t t = 0; t < (uint32_t)kBlock; ++t) {
+    const uint64_t chunk = block_chunk<kGroup>(k0 + t, c_lo);
+    if (chunk >= c_hi) break;
     const uint64_t e0 = chunk * kChunk;
     const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
     ChunkMap cm;
-    build_chunk_map(seg_off, P, e0, e1, cm, s_rel, s_pidx, s_scan, &s_cm);
+    build_chunk_map(seg_off, s_tab.first[t], s_tab.last[t], e0, cm, s_rel);
     if (STAGE && is_intersect(OP)) {
       // stage the largest backward list (<= kStageCap entries) that is bound
       // to the same vertex at both ends of the chunk
@@ -800,12 +1018,27 @@ __global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_
       }
     }
     uint64_t valid_count = 0;
+    if constexpr (is_intersect(OP) && !STAGE) {
+      // wavefront w owns the contiguous run [e0 + 512 w, e0 + 512 (w + 1))
+      bool valid[kItems];
+      const uint64_t base = e0 + (uint64_t)wave * (kItems * kWave);
+      acc += verify_run<OP>(a, seg_begin, seg_off, e0, base, e1, cm, s_rel, valid, path_counts);
+      if (kCompact) {
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+          const uint64_t bits = __ballot(valid[it]);
+          const uint64_t word_first = base + (uint64_t)it * kWave;
+          if (lane == 0 && word_first < W) mask[word_first / kWave] = bits;
+          valid_count += valid[it] ? 1 : 0;
+        }
+      }
+    } else
     for (int it = 0; it < kItems; ++it) {
       const uint64_t e = e0 + (uint64_t)it * kBlock + threadIdx.x;
       bool valid = false;
       if (e < e1) {
         uint64_t rank;
-        const uint64_t p = chunk_path(e, e0, cm, s_rel, s_pidx, seg_off, rank);
+        const uint64_t p = chunk_path(e, e0, cm, s_rel, seg_off, rank);
         const uint64_t packed = seg_begin[p];
         const uint64_t tag = packed >> 56;
         const uint32_t c = seg_source<OP>(a, tag)[(packed & kBeginMask) + rank];
@@ -829,6 +1062,7 @@ __global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_
     }
     __syncthreads();
   }
+  }
   if (!kCompact && OP != OP_IPATH) {
     uint64_t s = block_sum(acc, s_tmp);
     if (threadIdx.x == 0 && s) atomicAdd(total, (unsigned long long)s);
@@ -844,8 +1078,8 @@ __global__ __launch_bounds__(kBlock) void k_map(vsgm_level_args a, const uint64_
 // lane by lane with branchless binary lifting.
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_map_isect(vsgm_level_args a, const uint64_t* __restrict__ seg_begin,
-                                                      const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t n_chunks,
-                                                      uint64_t* __restrict__ mask, uint64_t* __restrict__ chunk_cnt,
+                                                      const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t n_chunks, uint64_t c_lo,
+                                                      uint64_t c_hi, uint64_t* __restrict__ mask, uint64_t* __restrict__ chunk_cnt,
                                                       unsigned long long* __restrict__ total, unsigned long long* __restrict__ path_counts) {
   __shared__ uint64_t s_off[kPathCap + 1];
   __shared__ uint64_t s_tmp[kBlock / kWave];
@@ -858,7 +1092,8 @@ __global__ __launch_bounds__(kBlock) void k_map_isect(vsgm_level_args a, const u
   const int wave = threadIdx.x / kWave;
   uint64_t acc = 0;
   uint64_t cursor_path = ~0ull;  // path whose list positions s_cursor[wave] holds
-  for (uint64_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+  (void)n_chunks;
+  for (uint64_t chunk = c_lo + blockIdx.x; chunk < c_hi; chunk += gridDim.x) {
     const uint64_t e0 = chunk * kChunk;
     const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
     ChunkPaths cp;
@@ -940,21 +1175,25 @@ __global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uin
                                                     const uint64_t* __restrict__ mask, const uint64_t* __restrict__ chunk_off,
                                                     uint32_t* __restrict__ out_vals, uint64_t* __restrict__ out_off, ExpandCols cols) {
   __shared__ int32_t s_rel[kPathCap + 1];
-  __shared__ uint16_t s_pidx[kChunk];
-  __shared__ uint32_t s_scan[kBlock / kWave];
   __shared__ uint64_t s_word[kWordsPerChunk + 1];
   __shared__ uint64_t s_wpre[kWordsPerChunk + 1];
-  __shared__ ChunkMap s_cm;
+  __shared__ ChunkTable s_tab;
+  constexpr bool kStarts = (OP != OP_EXPAND);
   const uint64_t P = a.path_num;
   const int lane = threadIdx.x & (kWave - 1);
-  for (uint64_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+  for (uint64_t k0 = 0; block_chunk<1>(k0) < n_chunks; k0 += kBlock) {
+  fill_chunk_table<1, kStarts>(seg_off, P, W, n_chunks, k0, &s_tab);
+  for (uint32_t t = 0; t < (uint32_t)kBlock; ++t) {
+    const uint64_t chunk = block_chunk<1>(k0 + t);
+    if (chunk >= n_chunks) break;
     const uint64_t e0 = chunk * kChunk;
     const uint64_t e1 = min(e0 + (uint64_t)kChunk, W);
     const uint32_t n_words = (uint32_t)((e1 - e0 + kWave - 1) / kWave);
     const uint64_t base = chunk_off[chunk];
     ChunkMap cm;
     if (threadIdx.x <= kWordsPerChunk) s_word[threadIdx.x] = threadIdx.x < n_words ? mask[e0 / kWave + threadIdx.x] : 0;
-    build_chunk_map(seg_off, P, e0, e1, cm, s_rel, s_pidx, s_scan, &s_cm);  // contains the barriers
+    build_chunk_map(seg_off, s_tab.first[t], s_tab.last[t], e0, cm, s_rel);
+    __syncthreads();
     if (threadIdx.x == 0) {
       uint64_t run = 0;
       for (uint32_t w = 0; w <= kWordsPerChunk; ++w) {
@@ -971,7 +1210,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uin
       if (!((word >> lane) & 1ull)) continue;
       const uint64_t pos = base + s_wpre[w] + __popcll(word & ((1ull << lane) - 1ull));
       uint64_t rank;
-      const uint64_t p = chunk_path(e, e0, cm, s_rel, s_pidx, seg_off, rank);
+      const uint64_t p = chunk_path(e, e0, cm, s_rel, seg_off, rank);
       const uint64_t packed = seg_begin[p];
       const uint32_t c = seg_source<OP>(a, packed >> 56)[(packed & kBeginMask) + rank];
       if (OP == OP_EXPAND) {
@@ -982,11 +1221,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uin
         out_vals[pos] = c;
       }
     }
-    if (OP != OP_EXPAND) {
+    if (kStarts) {
       // new row offsets of the paths whose segment starts in this chunk
-      const bool last_chunk = (chunk == n_chunks - 1);
-      const uint64_t p_lo = first_ge_u64(seg_off, P + 1, e0);
-      const uint64_t p_hi = last_chunk ? P + 1 : first_ge_u64(seg_off, P + 1, e1);
+      const uint64_t p_lo = s_tab.p_lo[t];
+      const uint64_t p_hi = s_tab.p_hi[t];
       for (uint64_t p = p_lo + threadIdx.x; p < p_hi; p += kBlock) {
         uint64_t rel = seg_off[p] - e0;
         uint32_t w = (uint32_t)(rel / kWave);
@@ -996,6 +1234,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(vsgm_level_args a, const uin
       }
     }
     __syncthreads();
+  }
   }
 }
 
@@ -1073,6 +1312,9 @@ struct vsgm_ctx {
   std::vector<KernelRecord> records;
   std::vector<hipEvent_t> spare_events;
   vsgm_tuning tuning{0, 8192};
+  // VSGM_PROGRESS=<seconds>: long map launches report their progress on
+  // stderr at most this often (0: silent)
+  double progress_s = 0;
 };
 
 namespace {
@@ -1328,28 +1570,68 @@ uint64_t map_bytes(const vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t W, ui
   return b;
 }
 
-// The intersection and two-set count operators use k_map_isect, the
-// candidate-streaming operators (expand, filter) k_map.
+// Chunks per map launch: a level op with more elements runs as several
+// launches over consecutive chunk windows (about 8.6e9 elements each), so
+// no single dispatch runs for minutes and progress can be reported between
+// windows.
+constexpr uint64_t kWindowChunks = 1ull << 22;
+
+// One map launch over chunks [c_lo, c_hi): k_map_isect or k_map, as the
+// tuning selects.
 template <int OP>
-void launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t n_chunks,
-                uint64_t* mask, uint64_t* chunk_cnt, unsigned long long* total, unsigned long long* path_counts) {
-  const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(n_chunks, 1), ctx->tuning.max_grid);
+void launch_map_window(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W,
+                       uint64_t n_chunks, uint64_t c_lo, uint64_t c_hi, uint64_t* mask, uint64_t* chunk_cnt, unsigned long long* total,
+                       unsigned long long* path_counts) {
+  const unsigned grid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(c_hi - c_lo, 1), ctx->tuning.max_grid);
   if constexpr (is_intersect(OP) || OP == OP_COUNT2) {
     if (ctx->tuning.isect_kernel == 1) {
-      hipLaunchKernelGGL(k_map_isect<OP>, dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask, chunk_cnt, total,
-                         path_counts);
+      hipLaunchKernelGGL(k_map_isect<OP>, dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, c_lo, c_hi, mask,
+                         chunk_cnt, total, path_counts);
       return;
     }
   }
   if constexpr (is_intersect(OP)) {
     if (ctx->tuning.isect_kernel == 2) {
-      hipLaunchKernelGGL((k_map<OP, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask, chunk_cnt,
-                         total, path_counts);
+      hipLaunchKernelGGL((k_map<OP, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, c_lo, c_hi, mask,
+                         chunk_cnt, total, path_counts);
       return;
     }
   }
-  hipLaunchKernelGGL(k_map<OP>, dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, mask, chunk_cnt, total,
-                     path_counts);
+  hipLaunchKernelGGL(k_map<OP>, dim3(grid), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W, n_chunks, c_lo, c_hi, mask, chunk_cnt,
+                     total, path_counts);
+}
+
+// The whole map of one level op, window by window. Each window is one
+// profiled launch carrying its share (by chunks) of the op's algorithmic
+// bytes `bytes`.
+template <int OP>
+int launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t n_chunks,
+               uint64_t bytes, uint64_t* mask, uint64_t* chunk_cnt, unsigned long long* total, unsigned long long* path_counts) {
+  const uint64_t n_windows = (n_chunks + kWindowChunks - 1) / kWindowChunks;
+  const bool report = ctx->progress_s > 0 && n_windows > 1;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto last = t0;
+  for (uint64_t w = 0; w < n_windows; ++w) {
+    const uint64_t c_lo = w * kWindowChunks, c_hi = std::min(n_chunks, c_lo + kWindowChunks);
+    const uint64_t e_lo = c_lo * kChunk, e_hi = std::min(W, c_hi * kChunk);
+    const uint64_t share = (uint64_t)((unsigned __int128)bytes * (c_hi - c_lo) / n_chunks);
+    {
+      ProfScope ps(ctx, kname<OP>(1), share, e_hi - e_lo, w == 0 ? a.path_num : 0);
+      launch_map_window<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, c_lo, c_hi, mask, chunk_cnt, total, path_counts);
+    }
+    VSGM_TRY(launch_check(ctx, kname<OP>(1)));
+    if (report) {
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - last).count() >= ctx->progress_s || w + 1 == n_windows) {
+        last = now;
+        fprintf(stderr, "vsgm: %s window %llu/%llu, %.3g of %.3g elements, %.1f s\n", kname<OP>(1), (unsigned long long)(w + 1),
+                (unsigned long long)n_windows, (double)e_hi, (double)W, std::chrono::duration<double>(now - t0).count());
+        fflush(stderr);
+      }
+    }
+  }
+  return VSGM_OK;
 }
 
 template <int OP>
@@ -1362,12 +1644,8 @@ int run_count(vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t* d_total, uint64
   ctx->stats.work = W;
   if (W == 0) return VSGM_OK;
   uint64_t n_chunks = (W + kChunk - 1) / kChunk;
-  {
-    ProfScope ps(ctx, kname<OP>(1), map_bytes<OP>(ctx, a, W, n_chunks), W, a.path_num);
-    launch_map<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, nullptr, nullptr, (unsigned long long*)d_total,
-                   (unsigned long long*)d_path_counts);
-  }
-  return launch_check(ctx, "k_map(count)");
+  return launch_map<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, map_bytes<OP>(ctx, a, W, n_chunks), nullptr, nullptr,
+                        (unsigned long long*)d_total, (unsigned long long*)d_path_counts);
 }
 
 // Compacting ops: write the valid elements (count of them: C).
@@ -1393,11 +1671,7 @@ int run_mark(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t W, uint
   uint64_t n_chunks = (W + kChunk - 1) / kChunk;
   VSGM_TRY(t.get((W + kWave - 1) / kWave, mask));
   VSGM_TRY(t.get(n_chunks + 1, chunk_off));
-  {
-    ProfScope ps(ctx, kname<OP>(1), map_bytes<OP>(ctx, a, W, n_chunks), W, a.path_num);
-    launch_map<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, *mask, *chunk_off, nullptr, nullptr);
-  }
-  VSGM_TRY(launch_check(ctx, "k_map(mark)"));
+  VSGM_TRY(launch_map<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, map_bytes<OP>(ctx, a, W, n_chunks), *mask, *chunk_off, nullptr, nullptr));
   VSGM_TRY(scan_impl(ctx, *chunk_off, *chunk_off, n_chunks));
   VSGM_TRY(read_u64(ctx, *chunk_off + n_chunks, count));
   ctx->stats.outputs = *count;
@@ -1487,6 +1761,7 @@ int vsgm_ctx_create(int device, uint64_t mem_budget, vsgm_ctx** out) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bail(VSGM_ERR_HIP);
   ctx->budget = mem_budget ? mem_budget : (uint64_t)(0.9 * (double)free_b);
+  if (const char* s = getenv("VSGM_PROGRESS")) ctx->progress_s = std::max(0.0, atof(s));
   *out = ctx;
   return VSGM_OK;
 }
@@ -1622,6 +1897,66 @@ int vsgm_fill_u64(vsgm_ctx* ctx, uint64_t* d, uint64_t n, uint64_t value) {
   if (n == 0) return VSGM_OK;
   hipLaunchKernelGGL(k_fill, dim3(grid_for((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d, n, value);
   return launch_check(ctx, "k_fill");
+}
+
+int vsgm_hub_index_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* h_hubs,
+                         uint32_t n_hubs, vsgm_hub_index* out) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (!out || !d_row_ptrs || !d_cols || (n_hubs && !h_hubs)) return fail(ctx, VSGM_ERR_ARG, "null hub index argument");
+  if (vertex_count == 0 || vertex_count > 0xFFFFFFFFull) return fail(ctx, VSGM_ERR_ARG, "vertex count out of range");
+  *out = vsgm_hub_index{nullptr, nullptr, 0, 0};
+  {
+    std::vector<bool> seen(vertex_count, false);
+    for (uint32_t i = 0; i < n_hubs; ++i) {
+      if (h_hubs[i] >= vertex_count || seen[h_hubs[i]]) return fail(ctx, VSGM_ERR_ARG, "hub %u is out of range or repeated", h_hubs[i]);
+      seen[h_hubs[i]] = true;
+    }
+  }
+  const uint64_t slot_pairs = (vertex_count + 1) / 2;          // u64 words holding the u32 slots
+  const uint64_t words = ((vertex_count + 63) / 64) * 2;       // u32 words per row, even
+  uint32_t *slot = nullptr, *bits = nullptr, *d_hubs = nullptr;
+  VSGM_TRY(pool_alloc(ctx, slot_pairs * sizeof(uint64_t), (void**)&slot));
+  auto undo = [&](int code) {
+    pool_free(ctx, slot);
+    pool_free(ctx, bits);
+    pool_free(ctx, d_hubs);
+    return code;
+  };
+  int s = pool_alloc(ctx, std::max<uint64_t>((uint64_t)n_hubs * words, 2) * sizeof(uint32_t), (void**)&bits);
+  if (s == VSGM_OK) s = pool_alloc(ctx, std::max<uint32_t>(n_hubs, 1) * sizeof(uint32_t), (void**)&d_hubs);
+  if (s != VSGM_OK) return undo(s);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for((slot_pairs + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, (uint64_t*)slot, slot_pairs,
+                     ~0ull);
+  if ((s = launch_check(ctx, "k_fill")) != VSGM_OK) return undo(s);
+  if (n_hubs) {
+    const uint64_t bit_pairs = (uint64_t)n_hubs * words / 2;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((bit_pairs + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, (uint64_t*)bits, bit_pairs,
+                       0ull);
+    if ((s = launch_check(ctx, "k_fill")) != VSGM_OK) return undo(s);
+    if (hipMemcpyAsync(d_hubs, h_hubs, (uint64_t)n_hubs * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      return undo(fail(ctx, VSGM_ERR_HIP, "hub list upload failed"));
+    hipLaunchKernelGGL(k_hub_slots, dim3(grid_for((n_hubs + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_hubs, n_hubs, slot);
+    if ((s = launch_check(ctx, "k_hub_slots")) != VSGM_OK) return undo(s);
+    hipLaunchKernelGGL(k_hub_bits, dim3(std::min<uint32_t>(n_hubs, (uint32_t)kMaxGrid)), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols,
+                       vertex_count, d_hubs, n_hubs, bits, words);
+    if ((s = launch_check(ctx, "k_hub_bits")) != VSGM_OK) return undo(s);
+  }
+  // the host hub list may be released once this returns
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return undo(fail(ctx, VSGM_ERR_HIP, "hub index build failed"));
+  pool_free(ctx, d_hubs);
+  out->slot = slot;
+  out->bits = bits;
+  out->words = words;
+  out->n_hubs = n_hubs;
+  return VSGM_OK;
+}
+
+int vsgm_hub_index_free(vsgm_ctx* ctx, vsgm_hub_index* index) {
+  if (!ctx || !index) return VSGM_ERR_ARG;
+  VSGM_TRY(pool_free(ctx, index->slot));
+  VSGM_TRY(pool_free(ctx, index->bits));
+  *index = vsgm_hub_index{nullptr, nullptr, 0, 0};
+  return VSGM_OK;
 }
 
 int vsgm_intersect_write(vsgm_ctx* ctx, const vsgm_level_args* a, vsgm_csr_out* out) {

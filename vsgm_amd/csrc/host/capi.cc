@@ -313,6 +313,8 @@ int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
     if (k == "batch_bytes") o.batch_bytes = (uint64_t)value;
     else if (k == "min_batch_bytes") o.min_batch_bytes = (uint64_t)value;
     else if (k == "profile") o.profile = value != 0;
+    else if (k == "hub_bytes") o.hub_bytes = value < 0 ? ~0ull : (uint64_t)value;
+    else if (k == "hub_min_degree") o.hub_min_degree = (uint64_t)value;
     else if (k == "isect_kernel" || k == "max_grid") {
       vsgm_tuning t;
       vsgm_get_tuning(e->e->Context(), &t);
@@ -321,6 +323,10 @@ int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
       if (vsgm_set_tuning(e->e->Context(), &t) != VSGM_OK) throw vsgm::Error("bad tuning value for " + k);
     } else throw vsgm::Error("unknown executor option " + k);
   });
+}
+
+int vsgm_executor_hub_count(const vsgm_executor* e, uint32_t* n) {
+  return Guard([&] { *n = e->e->HubCount(); });
 }
 
 int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h) {

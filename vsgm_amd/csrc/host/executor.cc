@@ -91,6 +91,7 @@ PipelineExecutor::PipelineExecutor(int device, uint64_t vertex_count, std::vecto
 }
 
 PipelineExecutor::~PipelineExecutor() {
+  vsgm_hub_index_free(ctx_, &hub_);
   d_row_ptrs_.reset();
   d_cols_.reset();
   d_sources_.reset();
@@ -135,6 +136,35 @@ void PipelineExecutor::Transfer(const ViewBinHolder& vbh) {
   d_sources_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources_num_, 1) * sizeof(uintV));
   Check(vsgm_h2d(ctx_, d_sources_->get(), vbh.GetSources().data(), sources_num_ * sizeof(uintV)), "sources upload");
   Check(vsgm_ctx_sync(ctx_), "view upload");
+  BuildHubIndex(vbh.GetRowPtrs());
+}
+
+// Index the highest-degree lists of the uploaded view: vertices of degree
+// >= hub_min_degree, largest first, as many as hub_bytes holds.
+void PipelineExecutor::BuildHubIndex(const uint64_t* h_row_ptrs) {
+  Check(vsgm_hub_index_free(ctx_, &hub_), "hub index release");
+  uint64_t limit = opt_.hub_bytes;
+  if (limit == ~0ull) {
+    uint64_t budget = 0;
+    vsgm_mem_info(ctx_, nullptr, &budget, nullptr);
+    limit = std::min<uint64_t>(budget / 8, 16ull << 30);
+  }
+  const uint64_t row_bytes = (vertex_count_ + 63) / 64 * 8;
+  const uint64_t cap = std::min<uint64_t>(limit / row_bytes, 0xFFFFFFFEull);
+  if (cap == 0 || vertex_count_ == 0) return;
+  std::vector<std::pair<uint64_t, uintV>> cand;
+  for (uint64_t v = 0; v < vertex_count_; ++v) {
+    const uint64_t d = h_row_ptrs[v + 1] - h_row_ptrs[v];
+    if (d >= std::max<uint64_t>(opt_.hub_min_degree, 1)) cand.emplace_back(d, (uintV)v);
+  }
+  if (cand.empty()) return;
+  const size_t k = (size_t)std::min<uint64_t>(cap, cand.size());
+  std::partial_sort(cand.begin(), cand.begin() + k, cand.end(), [](const auto& x, const auto& y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
+  std::vector<uint32_t> hubs(k);
+  for (size_t i = 0; i < k; ++i) hubs[i] = cand[i].second;
+  Check(vsgm_hub_index_build(ctx_, (const uint64_t*)d_row_ptrs_->get(), (const uint32_t*)d_cols_->get(), vertex_count_, hubs.data(),
+                             (uint32_t)k, &hub_),
+        "hub index build");
 }
 
 void PipelineExecutor::TransferGraph(const Graph& g, const std::vector<uintV>& sources) {
@@ -156,6 +186,7 @@ void PipelineExecutor::TransferGraph(const Graph& g, const std::vector<uintV>& s
   d_sources_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources_num_, 1) * sizeof(uintV));
   Check(vsgm_h2d(ctx_, d_sources_->get(), src->data(), sources_num_ * sizeof(uintV)), "sources upload");
   Check(vsgm_ctx_sync(ctx_), "graph upload");
+  BuildHubIndex(g.row_ptrs.data());
 }
 
 void PipelineExecutor::Match(bool do_match) {
@@ -193,6 +224,9 @@ void PipelineExecutor::FillArgs(size_t level, const Level& l, vsgm_level_args& a
   a.cols = (const uint32_t*)d_cols_->get();
   a.vertex_count = vertex_count_;
   a.path_num = l.path_num;
+  a.hub_slot = hub_.slot;
+  a.hub_bits = hub_.bits;
+  a.hub_words = hub_.words;
   for (int u = 0; u < kMaxQueryVerticesNum; ++u) {
     a.inst[u] = l.inst[u].ptr;
     a.cand[u] = l.cand[u].ptr;

@@ -36,6 +36,11 @@ struct ExecutorOptions {
   uint64_t batch_bytes = 0;     // 0: budget / remaining levels (BatchManager::GetSafeBatchSize)
   uint64_t min_batch_bytes = 128ull << 20;
   bool profile = false;         // time every level op with HIP events
+  // Dense membership index of the view's highest-degree lists (V/8 bytes
+  // each): at most hub_bytes of it (~0: min(budget/8, 16 GiB); 0: off), over
+  // vertices of view degree >= hub_min_degree.
+  uint64_t hub_bytes = ~0ull;
+  uint64_t hub_min_degree = 64;
 };
 
 class DeviceBuffer;
@@ -64,9 +69,11 @@ class PipelineExecutor {
   vsgm_ctx* Context() { return ctx_; }
   ExecutorOptions& Options() { return opt_; }
   uint64_t PeakDeviceBytes() const;
+  uint32_t HubCount() const { return hub_.n_hubs; }
 
  private:
   struct Level;
+  void BuildHubIndex(const uint64_t* h_row_ptrs);
   void ProcessLevel(size_t level, const Level& parent);
   void RunBatch(size_t level, const Level& slice);
   uint64_t LevelBatchBytes(size_t level) const;
@@ -82,6 +89,7 @@ class PipelineExecutor {
   std::shared_ptr<DeviceBuffer> d_row_ptrs_, d_cols_, d_sources_;
   uint64_t cols_size_ = 0;
   uint64_t sources_num_ = 0;
+  vsgm_hub_index hub_{nullptr, nullptr, 0, 0};
 
   const Plan* plan_ = nullptr;  // plan being matched
   size_t plan_index_ = 0;

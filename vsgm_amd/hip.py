@@ -50,6 +50,9 @@ class LevelArgs(C.Structure):
         ("cond", Cond * MAXQ),
         ("other", C.c_int32),
         ("other_op", C.c_uint32),
+        ("hub_slot", C.c_void_p),
+        ("hub_bits", C.c_void_p),
+        ("hub_words", C.c_uint64),
     ]
 
 
@@ -63,6 +66,10 @@ class ExpandOut(C.Structure):
 
 class OpStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("work", C.c_uint64), ("outputs", C.c_uint64), ("adj_entries", C.c_uint64)]
+
+
+class HubIndex(C.Structure):
+    _fields_ = [("slot", C.c_void_p), ("bits", C.c_void_p), ("words", C.c_uint64), ("n_hubs", C.c_uint32)]
 
 
 def _declare(L):
@@ -92,6 +99,8 @@ def _declare(L):
         "vsgm_children_count": ([P, C.POINTER(LevelArgs), i, P], i),
         "vsgm_split_batch": ([P, P, u64, u64, u64, u64, P, u64, C.POINTER(u64)], i),
         "vsgm_last_op_stats": ([P, C.POINTER(OpStats)], i),
+        "vsgm_hub_index_build": ([P, P, P, u64, P, C.c_uint32, C.POINTER(HubIndex)], i),
+        "vsgm_hub_index_free": ([P, C.POINTER(HubIndex)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -206,3 +215,14 @@ class DeviceContext:
         s = OpStats()
         self.check(hiplib().vsgm_last_op_stats(self.h, C.byref(s)), "stats")
         return s
+
+    def hub_index(self, row_ptrs: DeviceArray, cols: DeviceArray, vertex_count: int, hubs) -> HubIndex:
+        """Dense membership index of the lists of `hubs` (free with hub_index_free)."""
+        h = np.ascontiguousarray(hubs, dtype=np.uint32)
+        out = HubIndex()
+        self.check(hiplib().vsgm_hub_index_build(self.h, row_ptrs.ptr, cols.ptr, vertex_count, h.ctypes.data if h.size else None,
+                                                 h.size, C.byref(out)), "hub index")
+        return out
+
+    def hub_index_free(self, index: HubIndex):
+        self.check(hiplib().vsgm_hub_index_free(self.h, C.byref(index)), "hub index free")
