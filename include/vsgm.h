@@ -114,6 +114,24 @@ int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* con
 int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value);
 /* Number of view vertices in the current membership index. */
 int vsgm_executor_hub_count(const vsgm_executor* e, uint32_t* n);
+/* Launch shape and batch budget of one plan (0 = the executor option),
+ * as chosen by the autotuner (vsgm_amd/autotune.py). */
+int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_kernel, uint32_t max_grid, uint64_t batch_bytes);
+int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* isect_kernel, uint32_t* max_grid, uint64_t* batch_bytes);
+
+/* ---- batched mode (no counterpart in the reference, which uploads one
+ * view bin at a time and runs plans one after another with a host
+ * synchronisation per level op, pipeline_executor.h:64-109) -------------
+ * add_view keeps one more view bin resident next to the others (their
+ * index rows share the automatic index budget); match_all runs the listed
+ * plans (NULL = all) over every resident view; capture runs the same work
+ * once, recording the host-visible values, then captures it into one device
+ * graph; replay launches that graph (one launch for every plan and view) and
+ * adds its counts to the totals. Counts are identical to match_all. */
+int vsgm_executor_add_view(vsgm_executor* e, const vsgm_view_holder* h, int* index);
+int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans);
+int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint64_t* n_nodes);
+int vsgm_executor_replay(vsgm_executor* e, double* ms);
 int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h);
 /* Whole graph as one view; sources NULL = every vertex. */
 int vsgm_executor_transfer_graph(vsgm_executor* e, const vsgm_graph* g, const uint32_t* sources, uint64_t n_sources);

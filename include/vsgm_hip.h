@@ -213,6 +213,28 @@ typedef struct vsgm_tuning {
 int vsgm_set_tuning(vsgm_ctx* ctx, const vsgm_tuning* t);
 int vsgm_get_tuning(const vsgm_ctx* ctx, vsgm_tuning* t);
 
+/* ---- recorded replay and graph capture -----------------------------------
+ * No counterpart in the reference, whose host side synchronises on every
+ * level op (cudaMemcpy of totals in the src/executor/ops headers). The host decisions
+ * of a Match depend only on a few values read back from the device (element
+ * totals, output counts, batch ends). vsgm_tape_record .. vsgm_tape_stop
+ * records them during an ordinary run; between vsgm_capture_begin and
+ * vsgm_capture_end the same calls replay them without any device
+ * synchronisation while the context stream is captured into one graph,
+ * which vsgm_graph_launch then replays (same inputs -> same work, fresh
+ * results). The captured work must issue the recorded sequence exactly;
+ * capture_end fails otherwise. Buffers the graph reads but does not write
+ * first (views, sources, index) must outlive it; vsgm_graph_release drops it.
+ * vsgm_d2h_async enqueues a copy without waiting (for pinned destinations
+ * inside a capture). */
+int vsgm_d2h_async(vsgm_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int vsgm_tape_record(vsgm_ctx* ctx);
+int vsgm_tape_stop(vsgm_ctx* ctx, uint64_t* n_values);
+int vsgm_capture_begin(vsgm_ctx* ctx);
+int vsgm_capture_end(vsgm_ctx* ctx, uint64_t* n_nodes);
+int vsgm_graph_launch(vsgm_ctx* ctx);
+int vsgm_graph_release(vsgm_ctx* ctx);
+
 /* Turn profiling on/off; either way the records collected so far are dropped. */
 int vsgm_profile_enable(vsgm_ctx* ctx, int enable);
 /* Aggregate the records by kernel name (synchronises). */

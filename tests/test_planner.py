@@ -136,3 +136,42 @@ def test_hop_and_root_degree_of_baseline_patterns(vsgm):
     for spec, (hop, rd) in want.items():
         p = vsgm.Plan.optimize(_query_for(vsgm, spec))
         assert (p.hop, p.root_degree) == (hop, rd), spec
+
+
+def test_from_order_hop_covers_every_list_it_reads(vsgm, oracle):
+    """Over view bins (each a ball of radius hop-1 around its sources) the
+    per-bin counts depend on the root, but their sum must be the pattern's
+    count for every matching order: an order that reads the list of a
+    deeper vertex needs a larger hop than the breadth-first one."""
+    import itertools
+
+    g = vsgm.Graph.rmat(8, 16, seed=4)
+    rp, cl = g.csr()
+    path4 = np.zeros((4, 4), np.uint8)
+    for a, b in ((0, 1), (1, 2), (2, 3)):
+        path4[a, b] = path4[b, a] = 1
+    for adj in (path4, [s[2] for s in all_shapes((5,))][7]):
+        q = vsgm.Query.from_matrix(adj)
+        n, _, a, _ = q.info()
+        orders = [p for p in itertools.permutations(range(n))
+                  if all(any(a[p[i], p[j]] for j in range(i)) for i in range(1, n))][::3]
+        plans = [vsgm.Plan.from_order(q, o) for o in orders]
+        want = oracle.count_plan(rp, cl, plans[0].serialize())
+        hop = max(p.hop for p in plans)
+        rd = min(p.root_degree for p in plans)
+        assign = np.random.default_rng(1).integers(0, 3, size=g.vertex_count).astype(np.int32)
+        vbm = vsgm.ViewBinManager.from_assignment(g, hop, 3, assign)
+        h = vsgm.ViewBinHolder(g.vertex_count, g.entries * 4 + 4)
+        tot = np.zeros(len(plans), np.int64)
+        for b in range(vbm.n_bins):
+            vbm.materialize(b, h, rd)
+            vr, vc, src = h.csr(g.vertex_count)
+            tot += [oracle.count_plan(vr, vc, p.serialize(), sources=src) for p in plans]
+        assert tot.tolist() == [want] * len(plans)
+    # on the 4-path some connected order reads the list of a vertex two steps
+    # from its root (e.g. an end vertex matched before its neighbour), which
+    # needs hop 3, one more than any breadth-first order of the path
+    q = vsgm.Query.from_matrix(path4)
+    n, _, a, _ = q.info()
+    assert max(vsgm.Plan.from_order(q, o).hop for o in itertools.permutations(range(4))
+               if all(any(a[o[i], o[j]] for j in range(i)) for i in range(1, 4))) == 3

@@ -126,6 +126,12 @@ def _declare(L):
         "vsgm_executor_level_stats": ([P, C.c_int, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_peak_bytes": ([P, C.POINTER(u64)], C.c_int),
         "vsgm_executor_hub_count": ([P, C.POINTER(C.c_uint32)], C.c_int),
+        "vsgm_executor_set_plan_tuning": ([P, C.c_int, C.c_uint32, C.c_uint32, u64], C.c_int),
+        "vsgm_executor_get_plan_tuning": ([P, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(u64)], C.c_int),
+        "vsgm_executor_add_view": ([P, P, C.POINTER(C.c_int)], C.c_int),
+        "vsgm_executor_match_all": ([P, P, C.c_int], C.c_int),
+        "vsgm_executor_capture": ([P, P, C.c_int, C.POINTER(u64)], C.c_int),
+        "vsgm_executor_replay": ([P, C.POINTER(C.c_double)], C.c_int),
         "vsgm_executor_context": ([P, PP], C.c_int),
         "vsgm_profile_enable": ([P, C.c_int], C.c_int),
         "vsgm_profile_collect": ([P, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
@@ -494,6 +500,46 @@ class PipelineExecutor(_Handle):
         b = u64()
         _check(lib().vsgm_executor_peak_bytes(self._h, C.byref(b)), "peak")
         return b.value
+
+    def set_plan_tuning(self, plan: int, isect_kernel: int = 0, max_grid: int = 8192, batch_bytes: int = 0):
+        _check(lib().vsgm_executor_set_plan_tuning(self._h, plan, isect_kernel, max_grid, batch_bytes), "plan tuning")
+
+    def plan_tuning(self, plan: int) -> dict:
+        k, g, b = C.c_uint32(), C.c_uint32(), u64()
+        _check(lib().vsgm_executor_get_plan_tuning(self._h, plan, C.byref(k), C.byref(g), C.byref(b)), "plan tuning")
+        return {"isect_kernel": k.value, "max_grid": g.value, "batch_bytes": b.value}
+
+    def add_view(self, holder: ViewBinHolder) -> int:
+        """Keep one more view bin resident (batched mode)."""
+        i = C.c_int()
+        _check(lib().vsgm_executor_add_view(self._h, holder.handle, C.byref(i)), "add_view")
+        return i.value
+
+    @staticmethod
+    def _plan_list(plans):
+        if plans is None:
+            return None, 0
+        a = np.ascontiguousarray(np.asarray(list(plans), dtype=np.int32))
+        return a, len(a)
+
+    def match_all(self, plans: Optional[Sequence[int]] = None):
+        """The listed plans (all when None) over every resident view."""
+        a, n = self._plan_list(plans)
+        _check(lib().vsgm_executor_match_all(self._h, _ptr(a) if a is not None else None, n), "match_all")
+
+    def capture(self, plans: Optional[Sequence[int]] = None) -> int:
+        """Record one match_all, then capture it into one device graph;
+        returns the graph's node count (the recorded run's counts are added)."""
+        a, n = self._plan_list(plans)
+        nodes = u64()
+        _check(lib().vsgm_executor_capture(self._h, _ptr(a) if a is not None else None, n, C.byref(nodes)), "capture")
+        return nodes.value
+
+    def replay(self) -> float:
+        """Launch the captured graph once; returns its device time in ms."""
+        ms = C.c_double()
+        _check(lib().vsgm_executor_replay(self._h, C.byref(ms)), "replay")
+        return ms.value
 
     def hub_count(self) -> int:
         """Vertices of the current view in the dense membership index."""

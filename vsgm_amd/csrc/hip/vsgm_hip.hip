@@ -705,7 +705,10 @@ struct ChunkMap {
 };
 
 // The chunks one workgroup processes, in increasing order: groups of kGroup
-// consecutive chunks (counted from c_lo) dealt round-robin to the workgroups.
+// consecutive chunks (counted from c_lo) dealt round-robin to the
+// workgroups. (Placing neighbouring chunks on one XCD, so that they share an
+// L2, measured no faster: the index rows and lists they share are served
+// from the device-wide cache level either way.)
 template <uint64_t kGroup>
 __device__ inline uint64_t block_chunk(uint64_t k, uint64_t c_lo = 0) {
   return c_lo + ((uint64_t)blockIdx.x + (k / kGroup) * gridDim.x) * kGroup + k % kGroup;
@@ -1315,6 +1318,17 @@ struct vsgm_ctx {
   // VSGM_PROGRESS=<seconds>: long map launches report their progress on
   // stderr at most this often (0: silent)
   double progress_s = 0;
+  // Host-visible values (element totals, output counts, batch ends) are the
+  // only data that steer the host side of a level op. Tape mode 1 records
+  // them as they are read back; mode 2 replays them without touching the
+  // device, which is what lets a whole Match be captured into one graph.
+  int tape_mode = 0;
+  std::vector<uint64_t> tape;
+  size_t tape_pos = 0;
+  bool capturing = false;
+  bool saved_profiling = false;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_nodes = 0;
 };
 
 namespace {
@@ -1367,6 +1381,9 @@ int pool_alloc(vsgm_ctx* ctx, uint64_t bytes, void** out) {
     hipError_t e = hipMalloc(&p, cls);
     if (e != hipSuccess) {
       (void)hipGetLastError();
+      // cached blocks may be referenced by a captured graph: keep them
+      if (ctx->capturing || ctx->graph_exec)
+        return fail(ctx, VSGM_ERR_OOM, "hipMalloc(%llu) failed while a captured graph holds the pool", (unsigned long long)cls);
       // give cached blocks back and retry once
       HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
       for (auto& kv : ctx->cached) (void)hipFree(kv.second);
@@ -1448,12 +1465,30 @@ struct ProfScope {
   }
 };
 
-int read_u64(vsgm_ctx* ctx, const uint64_t* d, uint64_t* h) {
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  *h = ctx->h_pinned[0];
+// Read n u64 values the host side needs from device memory: a synchronous
+// copy (recorded on the tape in mode 1), or the next n tape values in
+// replay mode 2, with no device work at all.
+int host_read(vsgm_ctx* ctx, const uint64_t* d, uint64_t* h, uint64_t n) {
+  if (n == 0) return VSGM_OK;
+  if (ctx->tape_mode == 2) {
+    if (ctx->tape_pos + n > ctx->tape.size()) return fail(ctx, VSGM_ERR_ARG, "replay ran past the recorded values");
+    std::copy(ctx->tape.begin() + ctx->tape_pos, ctx->tape.begin() + ctx->tape_pos + n, h);
+    ctx->tape_pos += n;
+    return VSGM_OK;
+  }
+  if (n <= 8) {
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    std::copy(ctx->h_pinned, ctx->h_pinned + n, h);
+  } else {
+    HIP_TRY(ctx, hipMemcpyAsync(h, d, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  if (ctx->tape_mode == 1) ctx->tape.insert(ctx->tape.end(), h, h + n);
   return VSGM_OK;
 }
+
+int read_u64(vsgm_ctx* ctx, const uint64_t* d, uint64_t* h) { return host_read(ctx, d, h, 1); }
 
 int launch_check(vsgm_ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();
@@ -1543,10 +1578,10 @@ int run_setup_scan(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t**
   VSGM_TRY(scan_impl(ctx, *seg_off, *seg_off, P));
   hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, ctx->stream, *seg_off + P, ctx->d_stats);
   VSGM_TRY(launch_check(ctx, "k_copy1"));
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, ctx->d_stats, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  *W = ctx->h_pinned[0];
-  ctx->stats.adj_entries = ctx->h_pinned[1];
+  uint64_t vals[2];
+  VSGM_TRY(host_read(ctx, ctx->d_stats, vals, 2));
+  *W = vals[0];
+  ctx->stats.adj_entries = vals[1];
   return VSGM_OK;
 }
 
@@ -1608,7 +1643,7 @@ template <int OP>
 int launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t n_chunks,
                uint64_t bytes, uint64_t* mask, uint64_t* chunk_cnt, unsigned long long* total, unsigned long long* path_counts) {
   const uint64_t n_windows = (n_chunks + kWindowChunks - 1) / kWindowChunks;
-  const bool report = ctx->progress_s > 0 && n_windows > 1;
+  const bool report = ctx->progress_s > 0 && n_windows > 1 && !ctx->capturing;
   const auto t0 = std::chrono::steady_clock::now();
   auto last = t0;
   for (uint64_t w = 0; w < n_windows; ++w) {
@@ -1721,6 +1756,12 @@ int run_csr(vsgm_ctx* ctx, const vsgm_level_args& a, vsgm_csr_out* out) {
 
 }  // namespace
 
+// Context services for the other device sources of this library (mlp.hip).
+namespace vsgm_internal {
+hipStream_t stream(vsgm_ctx* ctx) { return ctx->stream; }
+int fail_msg(vsgm_ctx* ctx, int code, const char* msg) { return fail(ctx, code, "%s", msg); }
+}  // namespace vsgm_internal
+
 // ---------------------------------------------------------------------------
 // C-ABI
 
@@ -1770,6 +1811,7 @@ int vsgm_ctx_destroy(vsgm_ctx* ctx) {
   if (!ctx) return VSGM_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   for (auto& kv : ctx->cached) (void)hipFree(kv.second);
   for (auto& kv : ctx->blocks) (void)hipFree(kv.first);
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
@@ -1826,6 +1868,98 @@ int vsgm_d2h(vsgm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   if (bytes == 0) return VSGM_OK;
   HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return VSGM_OK;
+}
+
+int vsgm_d2h_async(vsgm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (bytes == 0) return VSGM_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  return VSGM_OK;
+}
+
+int vsgm_tape_record(vsgm_ctx* ctx) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (ctx->capturing) return fail(ctx, VSGM_ERR_ARG, "tape record during a capture");
+  ctx->tape.clear();
+  ctx->tape_pos = 0;
+  ctx->tape_mode = 1;
+  return VSGM_OK;
+}
+
+int vsgm_tape_stop(vsgm_ctx* ctx, uint64_t* n_values) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (ctx->capturing) return fail(ctx, VSGM_ERR_ARG, "tape stop during a capture");
+  ctx->tape_mode = 0;
+  if (n_values) *n_values = ctx->tape.size();
+  return VSGM_OK;
+}
+
+int vsgm_capture_begin(vsgm_ctx* ctx) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (ctx->capturing || ctx->tape_mode == 1) return fail(ctx, VSGM_ERR_ARG, "capture already running or tape recording");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->graph_exec) {
+    HIP_TRY(ctx, hipGraphExecDestroy(ctx->graph_exec));
+    ctx->graph_exec = nullptr;
+  }
+  ctx->saved_profiling = ctx->profiling;
+  ctx->profiling = false;
+  ctx->tape_mode = 2;
+  ctx->tape_pos = 0;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    ctx->tape_mode = 0;
+    ctx->profiling = ctx->saved_profiling;
+    return fail(ctx, VSGM_ERR_HIP, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+  }
+  ctx->capturing = true;
+  return VSGM_OK;
+}
+
+int vsgm_capture_end(vsgm_ctx* ctx, uint64_t* n_nodes) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (!ctx->capturing) return fail(ctx, VSGM_ERR_ARG, "no capture running");
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->capturing = false;
+  ctx->profiling = ctx->saved_profiling;
+  const bool consumed = ctx->tape_pos == ctx->tape.size();
+  ctx->tape_mode = 0;
+  if (e != hipSuccess || !g) return fail(ctx, VSGM_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  size_t n = 0;
+  (void)hipGraphGetNodes(g, nullptr, &n);
+  if (!consumed) {
+    (void)hipGraphDestroy(g);
+    return fail(ctx, VSGM_ERR_ARG, "the captured work read %llu of %llu recorded values", (unsigned long long)ctx->tape_pos,
+                (unsigned long long)ctx->tape.size());
+  }
+  e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    ctx->graph_exec = nullptr;
+    return fail(ctx, VSGM_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  ctx->graph_nodes = n;
+  if (n_nodes) *n_nodes = n;
+  return VSGM_OK;
+}
+
+int vsgm_graph_launch(vsgm_ctx* ctx) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (!ctx->graph_exec) return fail(ctx, VSGM_ERR_ARG, "no captured graph");
+  HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  return VSGM_OK;
+}
+
+int vsgm_graph_release(vsgm_ctx* ctx) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (ctx->graph_exec) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipGraphExecDestroy(ctx->graph_exec));
+    ctx->graph_exec = nullptr;
+    ctx->graph_nodes = 0;
+  }
   return VSGM_OK;
 }
 
@@ -2096,7 +2230,7 @@ int vsgm_split_batch(vsgm_ctx* ctx, const uint64_t* d_children, uint64_t n, uint
                        d_ends);
     VSGM_TRY(launch_check(ctx, "k_split"));
     std::vector<uint64_t> raw(nb - 1);
-    VSGM_TRY(vsgm_d2h(ctx, raw.data(), d_ends, (nb - 1) * sizeof(uint64_t)));
+    VSGM_TRY(host_read(ctx, d_ends, raw.data(), nb - 1));
     uint64_t prev = 0;
     for (uint64_t e : raw) {
       if (e <= prev) e = prev + 1;  // every batch keeps at least one parent

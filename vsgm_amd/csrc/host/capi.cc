@@ -316,12 +316,61 @@ int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
     else if (k == "hub_bytes") o.hub_bytes = value < 0 ? ~0ull : (uint64_t)value;
     else if (k == "hub_min_degree") o.hub_min_degree = (uint64_t)value;
     else if (k == "isect_kernel" || k == "max_grid") {
+      // applies to every plan (the per-plan tuning of the autotuner refines it)
       vsgm_tuning t;
       vsgm_get_tuning(e->e->Context(), &t);
       if (k == "isect_kernel") t.isect_kernel = (uint32_t)value;
       else t.max_grid = (uint32_t)value;
       if (vsgm_set_tuning(e->e->Context(), &t) != VSGM_OK) throw vsgm::Error("bad tuning value for " + k);
+      for (size_t i = 0; i < e->e->GetTotalCounts().size(); ++i) {
+        vsgm::PlanTuning pt = e->e->GetPlanTuning(i);
+        pt.isect_kernel = t.isect_kernel;
+        pt.max_grid = t.max_grid;
+        e->e->SetPlanTuning(i, pt);
+      }
     } else throw vsgm::Error("unknown executor option " + k);
+  });
+}
+
+int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_kernel, uint32_t max_grid, uint64_t batch_bytes) {
+  return Guard([&] {
+    if (isect_kernel > 2 || max_grid == 0 || max_grid > (1u << 20)) throw vsgm::Error("bad plan tuning");
+    e->e->SetPlanTuning((size_t)plan, vsgm::PlanTuning{isect_kernel, max_grid, batch_bytes});
+  });
+}
+
+int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* isect_kernel, uint32_t* max_grid, uint64_t* batch_bytes) {
+  return Guard([&] {
+    const auto& t = e->e->GetPlanTuning((size_t)plan);
+    *isect_kernel = t.isect_kernel;
+    *max_grid = t.max_grid;
+    *batch_bytes = t.batch_bytes;
+  });
+}
+
+int vsgm_executor_add_view(vsgm_executor* e, const vsgm_view_holder* h, int* index) {
+  return Guard([&] {
+    size_t i = e->e->AddView(*h->h);
+    if (index) *index = (int)i;
+  });
+}
+
+int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans) {
+  return Guard([&] { e->e->MatchAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{}); });
+}
+
+int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint64_t* n_nodes) {
+  return Guard([&] {
+    uint64_t n = e->e->CaptureAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{});
+    if (n_nodes) *n_nodes = n;
+  });
+}
+
+int vsgm_executor_replay(vsgm_executor* e, double* ms) {
+  return Guard([&] {
+    const double before = e->e->ReplayMs();
+    e->e->ReplayGraph();
+    if (ms) *ms = e->e->ReplayMs() - before;
   });
 }
 

@@ -86,23 +86,43 @@ PipelineExecutor::PipelineExecutor(int device, uint64_t vertex_count, std::vecto
   int s = vsgm_ctx_create(device, opt_.mem_budget, &ctx_);
   if (s == VSGM_ERR_NO_DEVICE) throw Error("no HIP device is visible: the enumerator needs a GPU");
   if (s != VSGM_OK) throw Error("cannot create the device context on device " + std::to_string(device) + " (status " + std::to_string(s) + ")");
-  d_row_ptrs_ = std::make_shared<DeviceBuffer>(ctx_, (vertex_count_ + 1) * sizeof(uintE));
   SetPlans(std::move(plans));
 }
 
 PipelineExecutor::~PipelineExecutor() {
-  vsgm_hub_index_free(ctx_, &hub_);
-  d_row_ptrs_.reset();
-  d_cols_.reset();
-  d_sources_.reset();
-  if (d_count_) vsgm_free(ctx_, d_count_);
+  vsgm_graph_release(ctx_);
+  ClearViews();
+  if (d_counts_) vsgm_free(ctx_, d_counts_);
+  if (h_counts_) vsgm_host_free(h_counts_);
   vsgm_ctx_destroy(ctx_);
 }
 
 void PipelineExecutor::SetPlans(std::vector<const Plan*> plans) {
   plans_ = std::move(plans);
   for (auto* p : plans_) VSGM_CHECK(p && p->GetVertexCount() <= (size_t)kMaxQueryVerticesNum, "bad plan");
+  ReleaseGraph();
+  if (d_counts_) vsgm_free(ctx_, d_counts_);
+  if (h_counts_) vsgm_host_free(h_counts_);
+  d_counts_ = nullptr;
+  h_counts_ = nullptr;
+  Check(vsgm_alloc(ctx_, std::max<size_t>(plans_.size(), 1) * sizeof(uint64_t), (void**)&d_counts_), "count allocation");
+  Check(vsgm_host_alloc(std::max<size_t>(plans_.size(), 1) * sizeof(uint64_t), (void**)&h_counts_), "count staging allocation");
+  tuning_.assign(plans_.size(), PlanTuning{});
   ResetCounts();
+}
+
+void PipelineExecutor::SetPlanTuning(size_t plan, const PlanTuning& t) {
+  VSGM_CHECK(plan < plans_.size(), "plan index out of range");
+  VSGM_CHECK(t.isect_kernel <= 2 && t.max_grid > 0, "bad plan tuning");
+  ReleaseGraph();
+  tuning_[plan] = t;
+}
+
+void PipelineExecutor::ApplyTuning(size_t plan) {
+  const PlanTuning& t = tuning_[plan];
+  vsgm_tuning kt{t.isect_kernel, t.max_grid};
+  Check(vsgm_set_tuning(ctx_, &kt), "tuning");
+  batch_override_ = t.batch_bytes;
 }
 
 void PipelineExecutor::ResetCounts() {
@@ -122,58 +142,80 @@ void PipelineExecutor::Check(int status, const char* what) const {
   if (status != VSGM_OK) throw Error(std::string(what) + " failed: " + vsgm_last_error(ctx_));
 }
 
-void PipelineExecutor::Transfer(const ViewBinHolder& vbh) {
-  VSGM_CHECK(vbh.GetVertexCount() == vertex_count_, "view vertex count does not match the executor");
-  Check(vsgm_h2d(ctx_, d_row_ptrs_->get(), vbh.GetRowPtrs(), (vertex_count_ + 1) * sizeof(uintE)), "row_ptrs upload");
-  if (!d_cols_ || cols_size_ < vbh.GetTotalSize()) {
-    d_cols_.reset();
-    d_cols_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(vbh.GetTotalSize(), 1) * sizeof(uintV));
-    cols_size_ = vbh.GetTotalSize();
-  }
-  Check(vsgm_h2d(ctx_, d_cols_->get(), vbh.GetCols(), vbh.GetTotalSize() * sizeof(uintV)), "cols upload");
-  sources_num_ = vbh.GetSourcesNum();
-  d_sources_.reset();
-  d_sources_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources_num_, 1) * sizeof(uintV));
-  Check(vsgm_h2d(ctx_, d_sources_->get(), vbh.GetSources().data(), sources_num_ * sizeof(uintV)), "sources upload");
-  Check(vsgm_ctx_sync(ctx_), "view upload");
-  BuildHubIndex(vbh.GetRowPtrs());
+void PipelineExecutor::ClearViews() {
+  ReleaseGraph();
+  for (auto& v : views_) vsgm_hub_index_free(ctx_, &v.hub);
+  views_.clear();
+  cur_ = 0;
+  view_ = nullptr;
 }
 
-// Index the highest-degree lists of the uploaded view: vertices of degree
-// >= hub_min_degree, largest first, as many as hub_bytes holds.
-void PipelineExecutor::BuildHubIndex(const uint64_t* h_row_ptrs) {
-  Check(vsgm_hub_index_free(ctx_, &hub_), "hub index release");
-  uint64_t limit = opt_.hub_bytes;
-  if (limit == ~0ull) {
-    uint64_t budget = 0;
-    vsgm_mem_info(ctx_, nullptr, &budget, nullptr);
-    limit = std::min<uint64_t>(budget / 8, 16ull << 30);
-  }
+// Upload one view (row_ptrs over the full vertex range, cols, sources) and
+// index its highest-degree lists: vertices of degree >= hub_min_degree,
+// largest first, as many as hub_limit bytes hold.
+PipelineExecutor::View PipelineExecutor::UploadView(const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n_cols, const uint32_t* sources,
+                                                    uint64_t n_sources, uint64_t hub_limit) {
+  View v;
+  v.row_ptrs = std::make_shared<DeviceBuffer>(ctx_, (vertex_count_ + 1) * sizeof(uintE));
+  v.cols = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(n_cols, 1) * sizeof(uintV));
+  v.sources = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(n_sources, 1) * sizeof(uintV));
+  v.sources_num = n_sources;
+  Check(vsgm_h2d(ctx_, v.row_ptrs->get(), row_ptrs, (vertex_count_ + 1) * sizeof(uintE)), "row_ptrs upload");
+  Check(vsgm_h2d(ctx_, v.cols->get(), cols, n_cols * sizeof(uintV)), "cols upload");
+  Check(vsgm_h2d(ctx_, v.sources->get(), sources, n_sources * sizeof(uintV)), "sources upload");
+  Check(vsgm_ctx_sync(ctx_), "view upload");
   const uint64_t row_bytes = (vertex_count_ + 63) / 64 * 8;
-  const uint64_t cap = std::min<uint64_t>(limit / row_bytes, 0xFFFFFFFEull);
-  if (cap == 0 || vertex_count_ == 0) return;
+  const uint64_t cap = std::min<uint64_t>(hub_limit / row_bytes, 0xFFFFFFFEull);
+  if (cap == 0 || vertex_count_ == 0) return v;
   std::vector<std::pair<uint64_t, uintV>> cand;
-  for (uint64_t v = 0; v < vertex_count_; ++v) {
-    const uint64_t d = h_row_ptrs[v + 1] - h_row_ptrs[v];
-    if (d >= std::max<uint64_t>(opt_.hub_min_degree, 1)) cand.emplace_back(d, (uintV)v);
+  for (uint64_t u = 0; u < vertex_count_; ++u) {
+    const uint64_t d = row_ptrs[u + 1] - row_ptrs[u];
+    if (d >= std::max<uint64_t>(opt_.hub_min_degree, 1)) cand.emplace_back(d, (uintV)u);
   }
-  if (cand.empty()) return;
+  if (cand.empty()) return v;
   const size_t k = (size_t)std::min<uint64_t>(cap, cand.size());
-  std::partial_sort(cand.begin(), cand.begin() + k, cand.end(), [](const auto& x, const auto& y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
+  std::partial_sort(cand.begin(), cand.begin() + k, cand.end(),
+                    [](const auto& x, const auto& y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
   std::vector<uint32_t> hubs(k);
   for (size_t i = 0; i < k; ++i) hubs[i] = cand[i].second;
-  Check(vsgm_hub_index_build(ctx_, (const uint64_t*)d_row_ptrs_->get(), (const uint32_t*)d_cols_->get(), vertex_count_, hubs.data(),
-                             (uint32_t)k, &hub_),
+  Check(vsgm_hub_index_build(ctx_, (const uint64_t*)v.row_ptrs->get(), (const uint32_t*)v.cols->get(), vertex_count_, hubs.data(),
+                             (uint32_t)k, &v.hub),
         "hub index build");
+  return v;
+}
+
+namespace {
+// Index bytes per view: the option, or min(budget/8, 16 GiB) shared by
+// `views` resident views.
+uint64_t HubLimit(vsgm_ctx* ctx, uint64_t option, size_t views) {
+  if (option != ~0ull) return option;
+  uint64_t budget = 0;
+  vsgm_mem_info(ctx, nullptr, &budget, nullptr);
+  return std::min<uint64_t>(budget / 8, 16ull << 30) / std::max<size_t>(views, 1);
+}
+}  // namespace
+
+void PipelineExecutor::Transfer(const ViewBinHolder& vbh) {
+  VSGM_CHECK(vbh.GetVertexCount() == vertex_count_, "view vertex count does not match the executor");
+  ClearViews();
+  views_.push_back(UploadView(vbh.GetRowPtrs(), vbh.GetCols(), vbh.GetTotalSize(), vbh.GetSources().data(), vbh.GetSourcesNum(),
+                              HubLimit(ctx_, opt_.hub_bytes, 1)));
+  cur_ = 0;
+}
+
+size_t PipelineExecutor::AddView(const ViewBinHolder& vbh) {
+  VSGM_CHECK(vbh.GetVertexCount() == vertex_count_, "view vertex count does not match the executor");
+  ReleaseGraph();
+  // the automatic index size assumes up to 8 resident views
+  views_.push_back(UploadView(vbh.GetRowPtrs(), vbh.GetCols(), vbh.GetTotalSize(), vbh.GetSources().data(), vbh.GetSourcesNum(),
+                              HubLimit(ctx_, opt_.hub_bytes, 8)));
+  cur_ = views_.size() - 1;
+  return cur_;
 }
 
 void PipelineExecutor::TransferGraph(const Graph& g, const std::vector<uintV>& sources) {
   VSGM_CHECK(g.vertex_count == vertex_count_, "graph vertex count does not match the executor");
-  Check(vsgm_h2d(ctx_, d_row_ptrs_->get(), g.row_ptrs.data(), (vertex_count_ + 1) * sizeof(uintE)), "row_ptrs upload");
-  d_cols_.reset();
-  d_cols_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(g.cols.size(), 1) * sizeof(uintV));
-  cols_size_ = g.cols.size();
-  Check(vsgm_h2d(ctx_, d_cols_->get(), g.cols.data(), g.cols.size() * sizeof(uintV)), "cols upload");
+  ClearViews();
   std::vector<uintV> all;
   const std::vector<uintV>* src = &sources;
   if (sources.empty()) {
@@ -181,52 +223,142 @@ void PipelineExecutor::TransferGraph(const Graph& g, const std::vector<uintV>& s
     for (uint64_t v = 0; v < vertex_count_; ++v) all[v] = (uintV)v;
     src = &all;
   }
-  sources_num_ = src->size();
-  d_sources_.reset();
-  d_sources_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources_num_, 1) * sizeof(uintV));
-  Check(vsgm_h2d(ctx_, d_sources_->get(), src->data(), sources_num_ * sizeof(uintV)), "sources upload");
-  Check(vsgm_ctx_sync(ctx_), "graph upload");
-  BuildHubIndex(g.row_ptrs.data());
+  views_.push_back(UploadView(g.row_ptrs.data(), g.cols.data(), g.cols.size(), src->data(), src->size(), HubLimit(ctx_, opt_.hub_bytes, 1)));
+  cur_ = 0;
 }
 
 void PipelineExecutor::Match(bool do_match) {
   if (!do_match) return;
-  VSGM_CHECK(d_cols_ && d_sources_, "Match before Transfer");
-  if (!d_count_) Check(vsgm_alloc(ctx_, sizeof(uint64_t), (void**)&d_count_), "count allocation");
-  for (size_t i = 0; i < plans_.size(); ++i) {
-    plan_ = plans_[i];
-    plan_index_ = i;
-    Check(vsgm_fill_u64(ctx_, d_count_, 1, 0), "count reset");
-    Check(vsgm_timer_start(ctx_, 0), "timer");
-    ProcessLevel(0, Level{});
-    float ms = 0;
-    Check(vsgm_timer_stop(ctx_, 0, &ms), "timer");
-    uint64_t c = 0;
-    Check(vsgm_d2h(ctx_, &c, d_count_, sizeof(uint64_t)), "count readback");
-    total_counts_[i] += c;
-    match_ms_[i] += ms;
-  }
-  plan_ = nullptr;
+  VSGM_CHECK(!views_.empty(), "Match before Transfer");
+  MatchPlans({cur_}, {});
 }
 
-uint64_t PipelineExecutor::LevelBatchBytes(size_t level) const {
-  if (opt_.batch_bytes) return opt_.batch_bytes;
-  uint64_t available = 0;
-  vsgm_mem_info(ctx_, &available, nullptr, nullptr);
-  size_t remaining = plan_->GetExecuteOperations().size() - level;
-  uint64_t b = (uint64_t)(0.9 * (double)(available / std::max<size_t>(remaining, 1)));
-  return std::max(b, opt_.min_batch_bytes);
+void PipelineExecutor::MatchAll(const std::vector<int>& which) {
+  VSGM_CHECK(!views_.empty(), "Match before Transfer");
+  std::vector<size_t> all(views_.size());
+  for (size_t i = 0; i < all.size(); ++i) all[i] = i;
+  MatchPlans(all, which);
+}
+
+// Every selected plan over every selected view, counts in d_counts_. Eager
+// mode times each plan with HIP events and reads the counts back once;
+// capture mode (tape_mode_ 2) leaves a copy of the counts into the pinned
+// slots as the graph's last node.
+void PipelineExecutor::MatchPlans(const std::vector<size_t>& views, const std::vector<int>& which) {
+  std::vector<int> sel = which;
+  if (sel.empty())
+    for (size_t i = 0; i < plans_.size(); ++i) sel.push_back((int)i);
+  for (int i : sel) VSGM_CHECK(i >= 0 && (size_t)i < plans_.size(), "plan index out of range");
+  const bool capturing = tape_mode_ == 2;
+  Check(vsgm_fill_u64(ctx_, d_counts_, plans_.size(), 0), "count reset");
+  for (int i : sel) {
+    plan_ = plans_[i];
+    plan_index_ = i;
+    d_count_ = d_counts_ + i;
+    ApplyTuning(i);
+    if (!capturing) Check(vsgm_timer_start(ctx_, 0), "timer");
+    for (size_t v : views) {
+      view_ = &views_[v];
+      ProcessLevel(0, Level{});
+    }
+    if (!capturing) {
+      float ms = 0;
+      Check(vsgm_timer_stop(ctx_, 0, &ms), "timer");
+      match_ms_[i] += ms;
+    }
+  }
+  plan_ = nullptr;
+  view_ = nullptr;
+  if (capturing) {
+    Check(vsgm_d2h_async(ctx_, h_counts_, d_counts_, plans_.size() * sizeof(uint64_t)), "count copy");
+    return;
+  }
+  std::vector<uint64_t> c(plans_.size(), 0);
+  Check(vsgm_d2h(ctx_, c.data(), d_counts_, c.size() * sizeof(uint64_t)), "count readback");
+  for (int i : sel) total_counts_[i] += c[i];
+}
+
+uint64_t PipelineExecutor::CaptureAll(const std::vector<int>& which) {
+  VSGM_CHECK(!views_.empty(), "capture before Transfer");
+  ReleaseGraph();
+  batch_tape_.clear();
+  tape_mode_ = 1;
+  Check(vsgm_tape_record(ctx_), "tape record");
+  try {
+    MatchAll(which);
+  } catch (...) {
+    tape_mode_ = 0;
+    vsgm_tape_stop(ctx_, nullptr);
+    throw;
+  }
+  Check(vsgm_tape_stop(ctx_, nullptr), "tape stop");
+  tape_mode_ = 2;
+  batch_pos_ = 0;
+  Check(vsgm_capture_begin(ctx_), "capture begin");
+  uint64_t nodes = 0;
+  try {
+    MatchAll(which);
+  } catch (...) {
+    tape_mode_ = 0;
+    vsgm_capture_end(ctx_, nullptr);
+    vsgm_graph_release(ctx_);
+    throw;
+  }
+  tape_mode_ = 0;
+  Check(vsgm_capture_end(ctx_, &nodes), "capture end");
+  VSGM_CHECK(batch_pos_ == batch_tape_.size(), "captured work diverged from the recording");
+  graph_ready_ = true;
+  graph_plans_ = which;
+  return nodes;
+}
+
+void PipelineExecutor::ReplayGraph() {
+  VSGM_CHECK(graph_ready_, "no captured graph");
+  Check(vsgm_timer_start(ctx_, 0), "timer");
+  Check(vsgm_graph_launch(ctx_), "graph launch");
+  float ms = 0;
+  Check(vsgm_timer_stop(ctx_, 0, &ms), "timer");
+  replay_ms_ += ms;
+  std::vector<int> sel = graph_plans_;
+  if (sel.empty())
+    for (size_t i = 0; i < plans_.size(); ++i) sel.push_back((int)i);
+  for (int i : sel) total_counts_[i] += h_counts_[i];
+}
+
+void PipelineExecutor::ReleaseGraph() {
+  if (graph_ready_) vsgm_graph_release(ctx_);
+  graph_ready_ = false;
+}
+
+uint64_t PipelineExecutor::LevelBatchBytes(size_t level) {
+  if (tape_mode_ == 2) {
+    VSGM_CHECK(batch_pos_ < batch_tape_.size(), "captured work diverged from the recording");
+    return batch_tape_[batch_pos_++];
+  }
+  uint64_t b;
+  if (batch_override_) {
+    b = batch_override_;
+  } else if (opt_.batch_bytes) {
+    b = opt_.batch_bytes;
+  } else {
+    uint64_t available = 0;
+    vsgm_mem_info(ctx_, &available, nullptr, nullptr);
+    size_t remaining = plan_->GetExecuteOperations().size() - level;
+    b = std::max((uint64_t)(0.9 * (double)(available / std::max<size_t>(remaining, 1))), opt_.min_batch_bytes);
+  }
+  if (tape_mode_ == 1) batch_tape_.push_back(b);
+  return b;
 }
 
 void PipelineExecutor::FillArgs(size_t level, const Level& l, vsgm_level_args& a) const {
   memset(&a, 0, sizeof(a));
-  a.row_ptrs = (const uint64_t*)d_row_ptrs_->get();
-  a.cols = (const uint32_t*)d_cols_->get();
+  a.row_ptrs = (const uint64_t*)view_->row_ptrs->get();
+  a.cols = (const uint32_t*)view_->cols->get();
   a.vertex_count = vertex_count_;
   a.path_num = l.path_num;
-  a.hub_slot = hub_.slot;
-  a.hub_bits = hub_.bits;
-  a.hub_words = hub_.words;
+  a.hub_slot = view_->hub.slot;
+  a.hub_bits = view_->hub.bits;
+  a.hub_words = view_->hub.words;
   for (int u = 0; u < kMaxQueryVerticesNum; ++u) {
     a.inst[u] = l.inst[u].ptr;
     a.cand[u] = l.cand[u].ptr;
@@ -276,13 +408,14 @@ void PipelineExecutor::FillArgs(size_t level, const Level& l, vsgm_level_args& a
 void PipelineExecutor::ProcessLevel(size_t level, const Level& parent) {
   const auto& ops = plan_->GetExecuteOperations();
   if (level == ops.size()) return;
-  auto& stat = level_stats_[plan_index_][level];
+  LevelStat unused;  // a capture only replays work already counted
+  LevelStat& stat = tape_mode_ == 2 ? unused : level_stats_[plan_index_][level];
   const size_t remaining = ops.size() - level;
 
   if (level == 0) {
     // InitFirstLevel (materialize.h:11-18): root bound to the bin's sources,
     // split evenly by the level-0 cost (processor.h:207-213).
-    const uint64_t P = sources_num_;
+    const uint64_t P = view_->sources_num;
     if (P == 0) return;
     const uint64_t pf = sizeof(uintV) + (uint64_t)std::ceil(3.0 * sizeof(uint64_t) / remaining);
     const uint64_t bytes = LevelBatchBytes(0);
@@ -292,8 +425,8 @@ void PipelineExecutor::ProcessLevel(size_t level, const Level& parent) {
     for (uint64_t l = 0; l < P; l += per) {
       Level child;
       child.path_num = std::min(per, P - l);
-      child.inst[root].owner = d_sources_;
-      child.inst[root].ptr = (uint32_t*)d_sources_->get() + l;
+      child.inst[root].owner = view_->sources;
+      child.inst[root].ptr = (uint32_t*)view_->sources->get() + l;
       child.inst[root].size = child.path_num;
       stat.batches++;
       stat.paths += child.path_num;
@@ -372,11 +505,13 @@ void PipelineExecutor::RunBatch(size_t level, const Level& slice) {
   const auto& e = plan_->GetExecuteOperations()[level];
   const auto& mat = plan_->GetMaterializedVertices()[level];
   const auto& cu = plan_->GetComputedUnmaterializedVertices()[level];
-  auto& stat = level_stats_[plan_index_][level];
+  LevelStat unused;
+  LevelStat& stat = tape_mode_ == 2 ? unused : level_stats_[plan_index_][level];
   const uintV u = e.second;
   vsgm_level_args a;
   FillArgs(level, slice, a);
-  if (opt_.profile) Check(vsgm_timer_start(ctx_, 1), "timer");
+  const bool timed = opt_.profile && tape_mode_ != 2;
+  if (timed) Check(vsgm_timer_start(ctx_, 1), "timer");
 
   Level child;
   bool next = false;
@@ -434,7 +569,7 @@ void PipelineExecutor::RunBatch(size_t level, const Level& slice) {
   stat.work += st.work;
   stat.outputs += st.outputs;
   stat.adj_entries += st.adj_entries;
-  if (opt_.profile) {
+  if (timed) {
     float ms = 0;
     Check(vsgm_timer_stop(ctx_, 1, &ms), "timer");
     stat.ms += ms;

@@ -45,6 +45,13 @@ struct ExecutorOptions {
 
 class DeviceBuffer;
 
+// Launch-shape and budget choices for one plan (set by the autotuner).
+struct PlanTuning {
+  uint32_t isect_kernel = 0;
+  uint32_t max_grid = 8192;
+  uint64_t batch_bytes = 0;  // 0: the executor's option
+};
+
 class PipelineExecutor {
  public:
   PipelineExecutor(int device, uint64_t vertex_count, std::vector<const Plan*> plans, uint64_t max_sources, ExecutorOptions opt = {});
@@ -52,14 +59,32 @@ class PipelineExecutor {
   PipelineExecutor(const PipelineExecutor&) = delete;
   PipelineExecutor& operator=(const PipelineExecutor&) = delete;
 
-  // Upload a materialised view and its sources (pipeline_executor.h:64-81).
+  // Upload a materialised view and its sources (pipeline_executor.h:64-81);
+  // replaces every resident view.
   void Transfer(const ViewBinHolder& vbh);
   // Upload a whole graph as one view whose sources are `sources` (all
   // vertices when empty); equivalent to a single-bin partition with
-  // hop >= the graph diameter.
+  // hop >= the graph diameter. Replaces every resident view.
   void TransferGraph(const Graph& g, const std::vector<uintV>& sources = {});
+  // Keep one more view resident (all views stay in HBM together); returns
+  // its index and makes it the current view.
+  size_t AddView(const ViewBinHolder& vbh);
+  size_t ViewCount() const { return views_.size(); }
   // Enumerate every plan over the current view (pipeline_executor.h:83-109).
   void Match(bool do_match = true);
+  // Enumerate the plans in `which` (all when empty) over every resident view.
+  void MatchAll(const std::vector<int>& which = {});
+
+  // Batched mode: run MatchAll(which) once while recording the values the
+  // host reads back, then capture the same work into one device graph.
+  // Returns the graph's node count. ReplayGraph runs the graph once and adds
+  // its counts to the totals.
+  uint64_t CaptureAll(const std::vector<int>& which = {});
+  void ReplayGraph();
+  void ReleaseGraph();
+
+  void SetPlanTuning(size_t plan, const PlanTuning& t);
+  const PlanTuning& GetPlanTuning(size_t plan) const { return tuning_.at(plan); }
 
   const std::vector<uintC>& GetTotalCounts() const { return total_counts_; }
   const std::vector<std::vector<LevelStat>>& GetLevelStats() const { return level_stats_; }
@@ -69,16 +94,25 @@ class PipelineExecutor {
   vsgm_ctx* Context() { return ctx_; }
   ExecutorOptions& Options() { return opt_; }
   uint64_t PeakDeviceBytes() const;
-  uint32_t HubCount() const { return hub_.n_hubs; }
+  uint32_t HubCount() const { return views_.empty() ? 0 : views_[cur_].hub.n_hubs; }
 
  private:
   struct Level;
-  void BuildHubIndex(const uint64_t* h_row_ptrs);
+  struct View {
+    std::shared_ptr<DeviceBuffer> row_ptrs, cols, sources;
+    uint64_t sources_num = 0;
+    vsgm_hub_index hub{nullptr, nullptr, 0, 0};
+  };
+  void ClearViews();
+  View UploadView(const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n_cols, const uint32_t* sources, uint64_t n_sources,
+                  uint64_t hub_limit);
+  void MatchPlans(const std::vector<size_t>& views, const std::vector<int>& which);
   void ProcessLevel(size_t level, const Level& parent);
   void RunBatch(size_t level, const Level& slice);
-  uint64_t LevelBatchBytes(size_t level) const;
+  uint64_t LevelBatchBytes(size_t level);
   void FillArgs(size_t level, const Level& l, vsgm_level_args& a) const;
   void Check(int status, const char* what) const;
+  void ApplyTuning(size_t plan);
 
   int device_;
   uint64_t vertex_count_;
@@ -86,14 +120,31 @@ class PipelineExecutor {
   ExecutorOptions opt_;
   vsgm_ctx* ctx_ = nullptr;
 
-  std::shared_ptr<DeviceBuffer> d_row_ptrs_, d_cols_, d_sources_;
-  uint64_t cols_size_ = 0;
-  uint64_t sources_num_ = 0;
-  vsgm_hub_index hub_{nullptr, nullptr, 0, 0};
+  std::vector<View> views_;
+  size_t cur_ = 0;
+  const View* view_ = nullptr;  // view being matched
 
   const Plan* plan_ = nullptr;  // plan being matched
   size_t plan_index_ = 0;
-  uint64_t* d_count_ = nullptr;
+  uint64_t* d_counts_ = nullptr;  // u64 per plan
+  uint64_t* d_count_ = nullptr;   // the matched plan's slot of d_counts_
+  uint64_t* h_counts_ = nullptr;  // pinned landing slots of the graph's count copy
+  std::vector<PlanTuning> tuning_;
+  uint64_t batch_override_ = 0;   // the matched plan's batch budget (0: none)
+
+  // batched mode: 0 eager, 1 recording, 2 capturing
+  int tape_mode_ = 0;
+  std::vector<uint64_t> batch_tape_;  // level budgets seen while recording
+  size_t batch_pos_ = 0;
+  bool graph_ready_ = false;
+  std::vector<int> graph_plans_;
+  double replay_ms_ = 0;
+
+ public:
+  double ReplayMs() const { return replay_ms_; }
+
+ private:
+
   std::vector<uintC> total_counts_;
   std::vector<std::vector<LevelStat>> level_stats_;
   std::vector<double> match_ms_;

@@ -299,7 +299,30 @@ Plan Plan::FromOrder(const Query& query, const std::vector<uintV>& seq) {
   VSGM_CHECK(IsConnectedOrder(query, seq), "matching order must be a connected permutation");
   Plan p(query);
   p.root_ = seq[0];
-  p.hop_ = p.GetMinHopWithBFS(p.root_);
+  // The view must hold the adjacency list of every pattern vertex that an
+  // intersection reads, i.e. every vertex with a neighbour later in the
+  // order: hop >= 1 + its BFS depth from the root. A breadth-first order
+  // (the heuristic's) needs no more than GetMinHopWithBFS; other orders
+  // can read deeper lists.
+  std::vector<int> depth(p.vertex_count_, -1);
+  std::queue<uintV> bfs;
+  bfs.push(p.root_);
+  depth[p.root_] = 0;
+  while (!bfs.empty()) {
+    uintV v = bfs.front();
+    bfs.pop();
+    for (auto w : p.conn_[v])
+      if (depth[w] < 0) {
+        depth[w] = depth[v] + 1;
+        bfs.push(w);
+      }
+  }
+  int need = 0;
+  for (size_t i = 0; i < seq.size(); ++i)
+    for (size_t j = i + 1; j < seq.size(); ++j)
+      if (std::find(p.conn_[seq[i]].begin(), p.conn_[seq[i]].end(), seq[j]) != p.conn_[seq[i]].end())
+        need = std::max(need, depth[seq[i]] + 1);
+  p.hop_ = std::max(p.GetMinHopWithBFS(p.root_), need);
   p.root_degree_ = p.conn_[p.root_].size();
   p.seq_ = seq;
   p.Finish();

@@ -1,0 +1,163 @@
+"""Learned planner: an MLP cost model that picks a pattern's matching order.
+
+The heuristic planner (Plan.optimize, a restatement of src/query/plan.h:61-580)
+stays the fallback and the reference for counts: every order gives the same
+count, so a learned plan can only change the time.
+
+Records. For a (pattern, view, order) the plain enumerator runs the plan of
+that order (Plan.from_order) over the view and its device time is logged next
+to the heuristic order's time on the same view. The label is
+log(time(order) / time(heuristic order)); the model learns relative cost, so
+patterns and views of very different sizes share one scale.
+
+Features (IN_DIM = 110, all in [0, 1] or small):
+  64  the pattern's adjacency permuted into the matching order (8 x 8, zero padded)
+   8  backward-neighbour count of each position / 7
+  32  the view's degree histogram: fraction of view vertices with degree in
+      [2^b, 2^(b+1)) for b = 0..31 (isolated vertices excluded)
+   4  log2 of vertex count, list entries, sources and maximum degree, / 40
+   2  pattern size / 8, plan hop / 4
+
+Choice. All connected orders (up to `max_orders`, the heuristic's always
+included) are scored on the GPU in one batch; the lowest predicted relative
+cost wins only if it predicts at least `margin` (log units) of gain over the
+heuristic order, otherwise the heuristic plan is kept.
+"""
+import itertools
+import math
+import time
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import api as V
+from .mlp import CostModel
+
+IN_DIM = 110
+MAXQ = 8
+
+
+def connected_orders(adj: np.ndarray, limit: Optional[int] = None, seed: int = 0) -> List[Tuple[int, ...]]:
+    """Matching orders in which every vertex after the first has an earlier
+    neighbour (the orders Plan.from_order accepts)."""
+    n = adj.shape[0]
+    out = [p for p in itertools.permutations(range(n)) if all(any(adj[p[i], p[j]] for j in range(i)) for i in range(1, n))]
+    if limit is not None and len(out) > limit:
+        rng = np.random.default_rng(seed)
+        out = [out[i] for i in sorted(rng.choice(len(out), size=limit, replace=False))]
+    return out
+
+
+def view_stats(row_ptrs: np.ndarray, n_sources: int) -> np.ndarray:
+    """The 36 view features: degree histogram and log sizes."""
+    deg = np.diff(row_ptrs.astype(np.int64))
+    nz = deg[deg > 0]
+    hist = np.zeros(32, np.float64)
+    if nz.size:
+        b = np.floor(np.log2(nz)).astype(np.int64)
+        hist = np.bincount(np.clip(b, 0, 31), minlength=32).astype(np.float64) / nz.size
+    sizes = np.array([math.log2(len(deg) + 1), math.log2(int(row_ptrs[-1]) + 1), math.log2(n_sources + 1),
+                      math.log2(int(deg.max()) + 1 if deg.size else 1)]) / 40.0
+    return np.concatenate([hist, sizes]).astype(np.float32)
+
+
+def featurize(adj: np.ndarray, order: Sequence[int], hop: int, vstats: np.ndarray) -> np.ndarray:
+    n = adj.shape[0]
+    perm = np.zeros((MAXQ, MAXQ), np.float32)
+    o = list(order)
+    perm[:n, :n] = adj[np.ix_(o, o)]
+    back = np.zeros(MAXQ, np.float32)
+    for i in range(1, n):
+        back[i] = perm[i, :i].sum() / 7.0
+    return np.concatenate([perm.ravel(), back, vstats, [n / 8.0, hop / 4.0]]).astype(np.float32)
+
+
+def _time_plans(ex, repeats: int) -> List[float]:
+    best = None
+    for _ in range(repeats):
+        ex.reset()
+        ex.match()
+        ms = ex.match_ms
+        best = ms if best is None else [min(a, b) for a, b in zip(best, ms)]
+    return best
+
+
+def collect(graphs, queries: Sequence["V.Query"], bins: int = 2, max_orders: int = 24, repeats: int = 2, seed: int = 0,
+            log=None) -> Dict[str, np.ndarray]:
+    """Training records from the plain enumerator. `graphs` are Graph objects;
+    each is split into `bins` view bins (seeded uniform assignment) and every
+    query runs in every sampled order over every bin. Returns x, y (log time
+    ratio to the heuristic order), the raw times and group ids."""
+    xs, ys, ms_all, groups = [], [], [], []
+    gid = 0
+    rng = np.random.default_rng(seed)
+    for gi, g in enumerate(graphs):
+        assign = rng.integers(0, bins, size=g.vertex_count).astype(np.int32)
+        for q in queries:
+            n, _, adj, _ = q.info()
+            heur = V.Plan.optimize(q)
+            orders = connected_orders(adj, max_orders, seed=seed + gid)
+            if tuple(heur.order) not in orders:
+                orders.append(tuple(heur.order))
+            plans = [V.Plan.from_order(q, o) for o in orders]
+            hop = max(p.hop for p in plans)
+            rd = min(p.root_degree for p in plans)
+            vbm = V.ViewBinManager.from_assignment(g, hop, bins, assign)
+            holder = V.ViewBinHolder(g.vertex_count, g.entries * 4 + 4)
+            ex = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
+            totals = np.zeros(len(plans), np.uint64)
+            for b in range(vbm.n_bins):
+                vbm.materialize(b, holder, rd)
+                rp, _, src = holder.csr(g.vertex_count)
+                ex.transfer(holder)
+                ms = _time_plans(ex, repeats)
+                # per bin the count depends on which pattern vertex is the
+                # root (bins partition the roots); summed over bins it is the
+                # pattern's count for every order
+                totals += np.array(ex.total_counts, np.uint64)
+                vs = view_stats(rp, len(src))
+                h = orders.index(tuple(heur.order))
+                for o, p, t in zip(orders, plans, ms):
+                    xs.append(featurize(adj, o, p.hop, vs))
+                    ys.append(math.log((t + 1e-3) / (ms[h] + 1e-3)))
+                    ms_all.append(t)
+                    groups.append(gid)
+                gid += 1
+            if len(set(totals.tolist())) != 1:
+                raise RuntimeError(f"orders disagree on the count: {totals.tolist()}")
+            if log:
+                log(f"graph {gi} query n={n}: {len(orders)} orders x {vbm.n_bins} bins")
+    return {"x": np.array(xs, np.float32), "y": np.array(ys, np.float32), "ms": np.array(ms_all, np.float64),
+            "group": np.array(groups, np.int64)}
+
+
+def train(data: Dict[str, np.ndarray], epochs: int = 60, batch: int = 128, lr: float = 2e-3, seed: int = 0, ctx=None):
+    """Fit a CostModel on collected records; returns (model, per-epoch loss)."""
+    m = CostModel(IN_DIM, seed=seed, ctx=ctx)
+    losses = m.train(data["x"], data["y"], epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return m, losses
+
+
+class LearnedPlanner:
+    """Picks matching orders with a trained CostModel; falls back to the
+    heuristic plan when no order is predicted to beat it by `margin`."""
+
+    def __init__(self, model: CostModel, max_orders: int = 120, margin: float = 0.1):
+        self.model, self.max_orders, self.margin = model, max_orders, margin
+
+    def plan(self, query: "V.Query", vstats: np.ndarray):
+        n, _, adj, _ = query.info()
+        heur = V.Plan.optimize(query)
+        orders = connected_orders(adj, self.max_orders)
+        h = tuple(heur.order)
+        if h not in orders:
+            orders.append(h)
+        hops = [heur.hop if o == h else V.Plan.from_order(query, o).hop for o in orders]
+        x = np.stack([featurize(adj, o, hp, vstats) for o, hp in zip(orders, hops)])
+        pred = self.model.forward(x)
+        best = int(np.argmin(pred))
+        hi = orders.index(h)
+        if orders[best] == h or pred[best] > pred[hi] - self.margin:
+            return heur, {"learned": False, "pred_gain": float(pred[hi] - pred[best])}
+        return V.Plan.from_order(query, orders[best]), {"learned": True, "pred_gain": float(pred[hi] - pred[best]),
+                                                        "order": list(orders[best])}
