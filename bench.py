@@ -100,6 +100,11 @@ def main():
     ap.add_argument("--hub-min-degree", type=int, default=64)
     ap.add_argument("--isect-kernel", type=int, default=0)
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
+    ap.add_argument("--workload", choices=["single", "batched"], default="single",
+                    help="single: the 4-clique/diamond/4-cycle line; batched: many random patterns over k view bins, "
+                         "one-at-a-time vs batched+tuned vs learned+batched")
+    ap.add_argument("--n-patterns", type=int, default=64)
+    ap.add_argument("--bins", type=int, default=8)
     args = ap.parse_args()
     os.environ["VSGM_PROGRESS"] = str(args.progress)
 
@@ -107,6 +112,9 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by view bins "
               f"across GPUs with no data exchange; only the single-GPU share is built and measured here)", flush=True)
         sys.exit(2)
+
+    if args.workload == "batched":
+        return batched_main(args)
 
     import vsgm_amd as V
 
@@ -233,6 +241,140 @@ def main():
     if args.detail_out:
         with open(args.detail_out, "w") as f:
             json.dump({"line": line, "kernels": kstats, "levels": level_stats}, f, indent=1)
+    print(json.dumps(line), flush=True)
+
+
+def batched_main(args):
+    """Many patterns over k resident view bins (the batched configuration):
+      plain    one pattern at a time, one bin at a time, heuristic plans
+               (transfer each bin, then Match one plan after another);
+      batched  all bins resident, every plan autotuned, the whole
+               enumeration captured once into one device graph and replayed;
+      learned  as batched, with matching orders from the learned planner
+               (trained here on records from smaller seeded R-MAT graphs).
+    Every count is checked equal across the three. Times are the match phase
+    (host wall clock around Match calls / the graph launch)."""
+    import numpy as np
+
+    import vsgm_amd as V
+    from vsgm_amd import autotune, learned
+
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from shapes import all_shapes
+
+    assert V.device_count() > 0, "no GPU visible"
+    rng = np.random.default_rng(args.seed)
+    shapes = all_shapes()
+    pick = rng.integers(0, len(shapes), size=args.n_patterns)  # uniform over the non-isomorphic shapes
+    queries = [V.Query.from_matrix(shapes[i][2]) for i in pick]
+    heur = [V.Plan.optimize(q) for q in queries]
+
+    t = time.perf_counter()
+    g = V.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed)
+    log(f"R-MAT s{args.scale}: V={g.vertex_count} E={g.edge_count} maxdeg={g.max_degree} in {time.perf_counter() - t:.1f}s; "
+        f"{len(queries)} patterns of sizes {np.bincount([shapes[i][0] for i in pick]).tolist()}")
+
+    # learned planner: records on smaller graphs, 3..5-vertex shapes
+    t = time.perf_counter()
+    six = all_shapes((6,))
+    tq = [V.Query.from_matrix(s[2]) for s in all_shapes((3, 4, 5))]
+    tq += [V.Query.from_matrix(six[i][2]) for i in np.random.default_rng(args.seed + 1).choice(len(six), 24, replace=False)]
+    data = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (8, 9)], tq, bins=2, max_orders=12, repeats=1,
+                           seed=args.seed)
+    model, losses = learned.train(data, epochs=60, batch=128, lr=2e-3, seed=args.seed)
+    log(f"learned planner: {len(data['y'])} records, loss {losses[0]:.3f} -> {losses[-1]:.3f} in {time.perf_counter() - t:.1f}s")
+
+    def make_bins(plans):
+        hop = max(p.hop for p in plans)
+        rd = min(p.root_degree for p in plans)
+        assign = np.random.default_rng(args.seed + 7).integers(0, args.bins, size=g.vertex_count).astype(np.int32)
+        vbm = V.ViewBinManager.from_assignment(g, hop, args.bins, assign)
+        holders = []
+        for b in range(vbm.n_bins):
+            h = V.ViewBinHolder(g.vertex_count, g.entries * 4 + 4)
+            vbm.materialize(b, h, rd)
+            holders.append(h)
+        return vbm, holders
+
+    planner = learned.LearnedPlanner(model, margin=0.1)
+    vbm0, holders0 = make_bins(heur)
+    rp0, _, src0 = holders0[0].csr(g.vertex_count)
+    vstats = learned.view_stats(rp0, len(src0))
+    lplans, n_learned = [], 0
+    for q in queries:
+        p, info = planner.plan(q, vstats)
+        lplans.append(p)
+        n_learned += int(info["learned"])
+    log(f"learned planner picked a non-heuristic order for {n_learned} of {len(queries)} patterns")
+    vbm, holders = make_bins(heur + lplans)  # one view family serves every plan
+
+    # plain: one bin at a time, one plan per Match
+    ex = V.PipelineExecutor(0, g.vertex_count, heur, max_sources=vbm.max_sources)
+    plain_s = 0.0
+    for b, h in enumerate(holders):
+        ex.transfer(h)
+        for i in range(len(heur)):
+            t = time.perf_counter()
+            ex.match_all([i])
+            plain_s += time.perf_counter() - t
+        log(f"plain: bin {b} done, {plain_s:.2f}s so far")
+    plain_counts = ex.total_counts
+    del ex
+
+    def batched(plans, label):
+        # tune on the first bin alone (a sample of the roots), then keep every bin resident
+        tx = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
+        tx.add_view(holders[0])
+        t = time.perf_counter()
+        res = autotune.tune(tx, repeats=1, candidates={"isect_kernel": [0, 2], "max_grid": [8192], "batch_bytes": [0, 1 << 30]})
+        tune_s = time.perf_counter() - t
+        del tx
+        bx = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
+        for h in holders:
+            bx.add_view(h)
+        for i, r in res.items():
+            bx.set_plan_tuning(i, **r["config"])
+        t = time.perf_counter()
+        nodes = bx.capture()
+        cap_s = time.perf_counter() - t
+        bx.reset()
+        t = time.perf_counter()
+        bx.replay()
+        run_s = time.perf_counter() - t
+        counts = bx.total_counts
+        log(f"{label}: tuned in {tune_s:.1f}s ({sum(r['config'] != {'isect_kernel': 0, 'max_grid': 8192, 'batch_bytes': 0} for r in res.values())} "
+            f"plans off the default), captured {nodes} nodes in {cap_s:.1f}s, replay {run_s:.3f}s")
+        return counts, run_s, nodes
+
+    b_counts, batched_s, b_nodes = batched(heur, "batched")
+    l_counts, learned_s, l_nodes = batched(lplans, "learned+batched")
+    assert b_counts == plain_counts, "batched counts differ from the one-at-a-time counts"
+    assert l_counts == plain_counts, "learned-plan counts differ from the heuristic counts"
+    total = sum(plain_counts)
+    line = {
+        "metric": "embeddings/sec per pattern, 1 MI355X; learned vs heuristic plan, exact counts",
+        "value": total / learned_s,
+        "unit": "embeddings/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": learned_s * 1000.0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": f"synthetic: seeded R-MAT (seed {args.seed}) generated on the host",
+        "config": {
+            "workload": f"batched: {len(queries)} random connected 3-6-vertex patterns (seed {args.seed}) over "
+                        f"{vbm.n_bins} view bins of R-MAT scale {args.scale} ef {args.edge_factor}",
+            "scale": args.scale, "bins": vbm.n_bins, "patterns": len(queries),
+            "total_embeddings": total,
+            "plain_s": plain_s, "batched_s": batched_s, "learned_batched_s": learned_s,
+            "speedup_batched": plain_s / batched_s, "speedup_learned_batched": plain_s / learned_s,
+            "graph_nodes": [b_nodes, l_nodes], "learned_orders": n_learned,
+            "counts_identical": True,
+        },
+    }
     print(json.dumps(line), flush=True)
 
 
