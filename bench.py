@@ -100,9 +100,10 @@ def main():
     ap.add_argument("--hub-min-degree", type=int, default=64)
     ap.add_argument("--isect-kernel", type=int, default=0)
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
-    ap.add_argument("--workload", choices=["single", "batched"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "copy-calibration"], default="single",
                     help="single: the 4-clique/diamond/4-cycle line; batched: many random patterns over k view bins, "
-                         "one-at-a-time vs batched+tuned vs learned+batched")
+                         "one-at-a-time vs batched+tuned vs learned+batched; copy-calibration: kernels of known HBM "
+                         "traffic (a 2 GiB fill and a 2 GiB reduction) that scale the rocprof byte counters")
     ap.add_argument("--n-patterns", type=int, default=64)
     ap.add_argument("--bins", type=int, default=8)
     args = ap.parse_args()
@@ -115,6 +116,8 @@ def main():
 
     if args.workload == "batched":
         return batched_main(args)
+    if args.workload == "copy-calibration":
+        return calibration_main()
 
     import vsgm_amd as V
 
@@ -242,6 +245,29 @@ def main():
         with open(args.detail_out, "w") as f:
             json.dump({"line": line, "kernels": kstats, "levels": level_stats}, f, indent=1)
     print(json.dumps(line), flush=True)
+
+
+def calibration_main():
+    """k_fill writes and k_reduce reads exactly 2 GiB (2^28 u64) per
+    dispatch, three dispatches each: what rocprof's WRITE_SIZE / FETCH_SIZE
+    report for them, divided by 2 GiB, is the factor that turns those
+    counters into bytes for the other kernels."""
+    import numpy as np
+
+    from vsgm_amd.hip import DeviceContext, hiplib
+
+    n = 1 << 28
+    ctx = DeviceContext(0)
+    buf = ctx.empty(n, np.uint64)
+    tot = ctx.array(np.zeros(1, np.uint64))
+    for _ in range(3):
+        ctx.check(hiplib().vsgm_fill_u64(ctx.h, buf.ptr, n, 3), "fill")
+        ctx.sync()
+    for _ in range(3):
+        ctx.check(hiplib().vsgm_reduce_u64(ctx.h, buf.ptr, n, tot.ptr), "reduce")
+        ctx.sync()
+    print(json.dumps({"calibration": "k_fill writes / k_reduce reads", "bytes_per_dispatch": n * 8,
+                      "sum_check": int(tot.numpy()[0]) == 3 * 3 * n}), flush=True)
 
 
 def batched_main(args):
