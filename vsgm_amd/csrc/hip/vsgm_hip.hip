@@ -416,10 +416,31 @@ __global__ __launch_bounds__(kBlock) void k_hub_bits(const uint64_t* __restrict_
 // ---------------------------------------------------------------------------
 // level operators
 
+// Relative cost of testing one candidate against a list bound to a vertex
+// that changes from path to path (its index row or search path is cold:
+// one fetched line per test) versus one bound to the same vertex as the
+// neighbouring paths (hot in cache); streaming one pivot element costs 1.
+constexpr uint64_t kColdTest = 4;
+
+// Is the vertex bound to backward position i the same in a neighbouring path?
+__device__ inline bool stable_binding(const vsgm_level_args& a, uint32_t i, uint64_t p, uint32_t v) {
+  const uint32_t* col = a.inst[a.back[i]];
+  return (p > 0 && col[p - 1] == v) || (p + 1 < a.path_num && col[p + 1] == v);
+}
+
 // Per-path segment: begin (low 56 bits, with the pivot position or the
 // count side in the top byte) and length. For intersections, *adj_sum
 // accumulates sum over paths of the degrees of all backward vertices (the
 // merge-model list volume used for the algorithmic byte count).
+//
+// Pivot choice. The reference streams the shortest backward list
+// (ThreadChoosePivotLevel, gpu_check_constraints_common.h:11-23). Any
+// backward list gives the same candidates in the same (ascending) order, so
+// here the pivot minimises deg(pivot) * (1 + sum of the other lists' test
+// costs): a list whose vertex is shared with the neighbouring paths is
+// cheap to test against (weight 1), one that changes per path costs
+// kColdTest. This streams the fresh list and probes the shared one when the
+// fresh list is at most a few times longer.
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* __restrict__ seg_begin, uint64_t* __restrict__ seg_len,
                                                   unsigned long long* __restrict__ adj_sum) {
@@ -429,13 +450,21 @@ __global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* _
   for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kBlock) {
     uint64_t lo = 0, hi = 0, tag = 0;
     if (is_intersect(OP)) {
-      uint64_t best_deg = 0;
+      uint64_t w_total = 0;
+      if (a.n_back > 1)
+        for (uint32_t i = 0; i < a.n_back; ++i) {
+          const uint32_t v = a.inst[a.back[i]][p];
+          w_total += stable_binding(a, i, p, v) ? 1 : kColdTest;
+        }
+      uint64_t best_cost = 0;
       for (uint32_t i = 0; i < a.n_back; ++i) {
-        uint32_t v = a.inst[a.back[i]][p];
-        uint64_t b = a.row_ptrs[v], e = a.row_ptrs[v + 1];
+        const uint32_t v = a.inst[a.back[i]][p];
+        const uint64_t b = a.row_ptrs[v], e = a.row_ptrs[v + 1];
         deg_sum += e - b;
-        if (i == 0 || e - b < best_deg) {  // first minimum wins (ThreadChoosePivotLevel)
-          best_deg = e - b;
+        const uint64_t w = a.n_back > 1 ? (stable_binding(a, i, p, v) ? 1 : kColdTest) : 0;
+        const uint64_t cost = (e - b) * (1 + w_total - w);
+        if (i == 0 || cost < best_cost) {  // first minimum wins
+          best_cost = cost;
           lo = b;
           hi = e;
           tag = i;
