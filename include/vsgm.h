@@ -129,6 +129,16 @@ int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* is
  * graph; replay launches that graph (one launch for every plan and view) and
  * adds its counts to the totals. Counts are identical to match_all. */
 int vsgm_executor_add_view(vsgm_executor* e, const vsgm_view_holder* h, int* index);
+/* Device-built views (vsgm_view_build in vsgm_hip.h; replaces the host
+ * ViewBin::Materialize + upload, view_bin.h:49-142, pipeline_executor.h:64-81):
+ * set_graph keeps the whole graph resident, add_device_view builds and keeps
+ * the view of one bin (its sources, hop, root degree) without host work;
+ * view_csr copies a resident view back (row_ptrs u64[V+1]; cols up to
+ * cols_cap entries, *entries = its size). */
+int vsgm_executor_set_graph(vsgm_executor* e, const vsgm_graph* g);
+int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree,
+                                  int* index);
+int vsgm_executor_view_csr(const vsgm_executor* e, int view, uint64_t* row_ptrs, uint32_t* cols, uint64_t cols_cap, uint64_t* entries);
 int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans);
 int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint64_t* n_nodes);
 int vsgm_executor_replay(vsgm_executor* e, double* ms);

@@ -154,6 +154,23 @@ int vsgm_hub_index_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32
                          uint32_t n_hubs, vsgm_hub_index* out);
 int vsgm_hub_index_free(vsgm_ctx* ctx, vsgm_hub_index* index);
 
+/* View of one bin built on the device from the whole graph resident in
+ * HBM (replaces the host ViewBin::Materialize + ExtendFromSource,
+ * src/view/view_bin.h:49-142, and the view upload of
+ * PipelineExecutor::Transfer, pipeline_executor.h:64-81): the full adjacency
+ * list of every vertex within hop-1 BFS levels of a source whose degree
+ * reaches root_degree (0: every source), empty lists elsewhere, over the full
+ * vertex range. row_ptrs u64[V+1] and cols u32[entries] are pool blocks
+ * owned by the caller; vertices = vertices inside the view. */
+typedef struct vsgm_view_out {
+  uint64_t* row_ptrs;
+  uint32_t* cols;
+  uint64_t entries;
+  uint64_t vertices;
+} vsgm_view_out;
+int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* d_sources,
+                    uint64_t n_sources, int hop, uint64_t root_degree, vsgm_view_out* out);
+
 /* A candidate CSR produced by an op: offsets u64[path_num+1] and values
  * u32[count], both pool-allocated (free with vsgm_free). */
 typedef struct vsgm_csr_out {

@@ -129,6 +129,9 @@ def _declare(L):
         "vsgm_executor_set_plan_tuning": ([P, C.c_int, C.c_uint32, C.c_uint32, u64], C.c_int),
         "vsgm_executor_get_plan_tuning": ([P, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(u64)], C.c_int),
         "vsgm_executor_add_view": ([P, P, C.POINTER(C.c_int)], C.c_int),
+        "vsgm_executor_set_graph": ([P, P], C.c_int),
+        "vsgm_executor_add_device_view": ([P, P, u64, C.c_int, u64, C.POINTER(C.c_int)], C.c_int),
+        "vsgm_executor_view_csr": ([P, C.c_int, P, P, u64, C.POINTER(u64)], C.c_int),
         "vsgm_executor_match_all": ([P, P, C.c_int], C.c_int),
         "vsgm_executor_capture": ([P, P, C.c_int, C.POINTER(u64)], C.c_int),
         "vsgm_executor_replay": ([P, C.POINTER(C.c_double)], C.c_int),
@@ -514,6 +517,25 @@ class PipelineExecutor(_Handle):
         i = C.c_int()
         _check(lib().vsgm_executor_add_view(self._h, holder.handle, C.byref(i)), "add_view")
         return i.value
+
+    def set_graph(self, graph: Graph):
+        """Keep the whole graph resident for device-built views."""
+        _check(lib().vsgm_executor_set_graph(self._h, graph.handle), "set_graph")
+
+    def add_device_view(self, sources, hop: int, root_degree: int) -> int:
+        """Build one bin's view on the device (from the set_graph graph) and keep it resident."""
+        s = np.ascontiguousarray(np.asarray(sources, dtype=np.uint32))
+        i = C.c_int()
+        _check(lib().vsgm_executor_add_device_view(self._h, _ptr(s), len(s), hop, root_degree, C.byref(i)), "add_device_view")
+        return i.value
+
+    def view_csr(self, view: int, vertex_count: int):
+        n = u64()
+        rp = np.zeros(vertex_count + 1, np.uint64)
+        _check(lib().vsgm_executor_view_csr(self._h, view, _ptr(rp), None, 0, C.byref(n)), "view_csr")
+        cl = np.zeros(max(n.value, 1), np.uint32)
+        _check(lib().vsgm_executor_view_csr(self._h, view, None, _ptr(cl), cl.size, C.byref(n)), "view_csr")
+        return rp, cl[: n.value]
 
     @staticmethod
     def _plan_list(plans):

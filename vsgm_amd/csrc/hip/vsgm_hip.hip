@@ -503,6 +503,63 @@ __global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* _
 // back after a setup + scan into one 16-byte copy.
 __global__ void k_copy1(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst) { dst[0] = src[0]; }
 
+// ---------------------------------------------------------------------------
+// device view builder (the view of ViewBin::Materialize, built from the
+// graph resident in HBM): dist[v] = BFS level from the qualifying sources
+// (0xFF: outside), then the lists of every vertex within hop-1 levels.
+
+constexpr uint8_t kFar = 0xFF;
+
+__global__ __launch_bounds__(kBlock) void k_view_seed(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ sources,
+                                                      uint64_t n_sources, uint64_t root_degree, uint8_t* __restrict__ dist) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n_sources; i += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t s = sources[i];
+    if (root_degree == 0 || row_ptrs[s + 1] - row_ptrs[s] >= root_degree) dist[s] = 0;
+  }
+}
+
+// One wavefront per vertex of the previous level; lanes stride its list.
+// Several lanes may mark the same vertex: they all store the same level.
+__global__ __launch_bounds__(kBlock) void k_view_expand(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ cols,
+                                                        uint64_t V, uint8_t level, uint8_t* __restrict__ dist) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t v = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < V; v += waves) {
+    if (dist[v] != (uint8_t)(level - 1)) continue;
+    for (uint64_t j = row_ptrs[v] + lane; j < row_ptrs[v + 1]; j += kWave) {
+      const uint32_t w = cols[j];
+      if (dist[w] == kFar) dist[w] = level;
+    }
+  }
+}
+
+// len[v] = deg(v) inside the view, 0 outside; *n_in += vertices inside.
+__global__ __launch_bounds__(kBlock) void k_view_len(const uint64_t* __restrict__ row_ptrs, const uint8_t* __restrict__ dist, uint64_t V,
+                                                     uint64_t* __restrict__ len, unsigned long long* __restrict__ n_in) {
+  __shared__ uint64_t s_tmp[kBlock / kWave];
+  uint64_t cnt = 0;
+  for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+    const bool in = dist[v] != kFar;
+    len[v] = in ? row_ptrs[v + 1] - row_ptrs[v] : 0;
+    cnt += in ? 1 : 0;
+  }
+  const uint64_t s = block_sum(cnt, s_tmp);
+  if (threadIdx.x == 0 && s) atomicAdd(n_in, (unsigned long long)s);
+}
+
+// One wavefront per vertex inside the view copies its list (coalesced).
+__global__ __launch_bounds__(kBlock) void k_view_copy(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ cols,
+                                                      const uint8_t* __restrict__ dist, uint64_t V, const uint64_t* __restrict__ vrp,
+                                                      uint32_t* __restrict__ vcols) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  for (uint64_t v = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < V; v += waves) {
+    if (dist[v] == kFar) continue;
+    const uint64_t b = row_ptrs[v], n = row_ptrs[v + 1] - b, o = vrp[v];
+    for (uint64_t j = lane; j < n; j += kWave) vcols[o + j] = cols[b + j];
+  }
+}
+
 struct ExpandCols {
   uint32_t n_mat;
   uint32_t mat[VSGM_MAX_QV];
@@ -2111,6 +2168,61 @@ int vsgm_hub_index_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32
   out->bits = bits;
   out->words = words;
   out->n_hubs = n_hubs;
+  return VSGM_OK;
+}
+
+int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* d_sources,
+                    uint64_t n_sources, int hop, uint64_t root_degree, vsgm_view_out* out) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (!out || !d_row_ptrs || !d_cols || (n_sources && !d_sources)) return fail(ctx, VSGM_ERR_ARG, "null view argument");
+  if (hop < 1 || hop > 250) return fail(ctx, VSGM_ERR_ARG, "hop out of range");
+  if (vertex_count == 0 || vertex_count > 0xFFFFFFFFull) return fail(ctx, VSGM_ERR_ARG, "vertex count out of range");
+  *out = vsgm_view_out{nullptr, nullptr, 0, 0};
+  const uint64_t V = vertex_count;
+  Temps t(ctx);
+  uint8_t* dist;
+  uint64_t* vrp = nullptr;
+  VSGM_TRY(t.get((V + 7) / 8 * 8, &dist));
+  VSGM_TRY(pool_alloc(ctx, (V + 1) * sizeof(uint64_t), (void**)&vrp));
+  auto undo = [&](int code) {
+    pool_free(ctx, vrp);
+    return code;
+  };
+  int s;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(((V + 7) / 8 + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, (uint64_t*)dist, (V + 7) / 8,
+                     ~0ull);
+  if ((s = launch_check(ctx, "k_fill")) != VSGM_OK) return undo(s);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stats, 2, 0);
+  if ((s = launch_check(ctx, "k_fill")) != VSGM_OK) return undo(s);
+  if (n_sources) {
+    hipLaunchKernelGGL(k_view_seed, dim3(grid_for((n_sources + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_sources,
+                       n_sources, root_degree, dist);
+    if ((s = launch_check(ctx, "k_view_seed")) != VSGM_OK) return undo(s);
+  }
+  const unsigned wave_grid = grid_for((V + kBlock / kWave - 1) / (kBlock / kWave));
+  for (int level = 1; level < hop; ++level) {
+    hipLaunchKernelGGL(k_view_expand, dim3(wave_grid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, V, (uint8_t)level, dist);
+    if ((s = launch_check(ctx, "k_view_expand")) != VSGM_OK) return undo(s);
+  }
+  hipLaunchKernelGGL(k_view_len, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_row_ptrs, dist, V, vrp,
+                     (unsigned long long*)(ctx->d_stats + 1));
+  if ((s = launch_check(ctx, "k_view_len")) != VSGM_OK) return undo(s);
+  if ((s = scan_impl(ctx, vrp, vrp, V)) != VSGM_OK) return undo(s);
+  hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, ctx->stream, vrp + V, ctx->d_stats);
+  if ((s = launch_check(ctx, "k_copy1")) != VSGM_OK) return undo(s);
+  uint64_t vals[2];
+  if ((s = host_read(ctx, ctx->d_stats, vals, 2)) != VSGM_OK) return undo(s);
+  uint32_t* vcols = nullptr;
+  if ((s = pool_alloc(ctx, std::max<uint64_t>(vals[0], 1) * sizeof(uint32_t), (void**)&vcols)) != VSGM_OK) return undo(s);
+  hipLaunchKernelGGL(k_view_copy, dim3(wave_grid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, dist, V, vrp, vcols);
+  if ((s = launch_check(ctx, "k_view_copy")) != VSGM_OK) {
+    pool_free(ctx, vcols);
+    return undo(s);
+  }
+  out->row_ptrs = vrp;
+  out->cols = vcols;
+  out->entries = vals[0];
+  out->vertices = vals[1];
   return VSGM_OK;
 }
 

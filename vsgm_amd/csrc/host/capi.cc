@@ -355,6 +355,33 @@ int vsgm_executor_add_view(vsgm_executor* e, const vsgm_view_holder* h, int* ind
   });
 }
 
+int vsgm_executor_set_graph(vsgm_executor* e, const vsgm_graph* g) {
+  return Guard([&] { e->e->SetGraph(g->g); });
+}
+
+int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree,
+                                  int* index) {
+  return Guard([&] {
+    std::vector<vsgm::uintV> s(sources, sources + n_sources);
+    size_t i = e->e->AddDeviceView(s, hop, root_degree);
+    if (index) *index = (int)i;
+  });
+}
+
+int vsgm_executor_view_csr(const vsgm_executor* e, int view, uint64_t* row_ptrs, uint32_t* cols, uint64_t cols_cap, uint64_t* entries) {
+  return Guard([&] {
+    std::vector<uint64_t> rp;
+    std::vector<uint32_t> cl;
+    e->e->ViewCsr((size_t)view, rp, cl);
+    if (entries) *entries = cl.size();
+    if (row_ptrs) std::copy(rp.begin(), rp.end(), row_ptrs);
+    if (cols) {
+      if (cl.size() > cols_cap) throw vsgm::Error("view has more entries than the buffer holds");
+      std::copy(cl.begin(), cl.end(), cols);
+    }
+  });
+}
+
 int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans) {
   return Guard([&] { e->e->MatchAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{}); });
 }

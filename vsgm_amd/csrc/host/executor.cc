@@ -43,6 +43,15 @@ Column<T> Adopt(vsgm_ctx* ctx, T* p, uint64_t size) {
 }
 
 bool Contains(const VTGroup& g, uintV u) { return std::find(g.begin(), g.end(), u) != g.end(); }
+
+// Index bytes per view: the option, or min(budget/8, 16 GiB) shared by
+// `views` resident views.
+uint64_t HubLimit(vsgm_ctx* ctx, uint64_t option, size_t views) {
+  if (option != ~0ull) return option;
+  uint64_t budget = 0;
+  vsgm_mem_info(ctx, nullptr, &budget, nullptr);
+  return std::min<uint64_t>(budget / 8, 16ull << 30) / std::max<size_t>(views, 1);
+}
 }  // namespace
 
 struct PipelineExecutor::Level {
@@ -92,6 +101,8 @@ PipelineExecutor::PipelineExecutor(int device, uint64_t vertex_count, std::vecto
 PipelineExecutor::~PipelineExecutor() {
   vsgm_graph_release(ctx_);
   ClearViews();
+  graph_rp_.reset();
+  graph_cols_.reset();
   if (d_counts_) vsgm_free(ctx_, d_counts_);
   if (h_counts_) vsgm_host_free(h_counts_);
   vsgm_ctx_destroy(ctx_);
@@ -164,15 +175,20 @@ PipelineExecutor::View PipelineExecutor::UploadView(const uint64_t* row_ptrs, co
   Check(vsgm_h2d(ctx_, v.cols->get(), cols, n_cols * sizeof(uintV)), "cols upload");
   Check(vsgm_h2d(ctx_, v.sources->get(), sources, n_sources * sizeof(uintV)), "sources upload");
   Check(vsgm_ctx_sync(ctx_), "view upload");
+  IndexView(v, row_ptrs, hub_limit);
+  return v;
+}
+
+void PipelineExecutor::IndexView(View& v, const uint64_t* row_ptrs, uint64_t hub_limit) {
   const uint64_t row_bytes = (vertex_count_ + 63) / 64 * 8;
   const uint64_t cap = std::min<uint64_t>(hub_limit / row_bytes, 0xFFFFFFFEull);
-  if (cap == 0 || vertex_count_ == 0) return v;
+  if (cap == 0 || vertex_count_ == 0) return;
   std::vector<std::pair<uint64_t, uintV>> cand;
   for (uint64_t u = 0; u < vertex_count_; ++u) {
     const uint64_t d = row_ptrs[u + 1] - row_ptrs[u];
     if (d >= std::max<uint64_t>(opt_.hub_min_degree, 1)) cand.emplace_back(d, (uintV)u);
   }
-  if (cand.empty()) return v;
+  if (cand.empty()) return;
   const size_t k = (size_t)std::min<uint64_t>(cap, cand.size());
   std::partial_sort(cand.begin(), cand.begin() + k, cand.end(),
                     [](const auto& x, const auto& y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
@@ -181,19 +197,47 @@ PipelineExecutor::View PipelineExecutor::UploadView(const uint64_t* row_ptrs, co
   Check(vsgm_hub_index_build(ctx_, (const uint64_t*)v.row_ptrs->get(), (const uint32_t*)v.cols->get(), vertex_count_, hubs.data(),
                              (uint32_t)k, &v.hub),
         "hub index build");
-  return v;
 }
 
-namespace {
-// Index bytes per view: the option, or min(budget/8, 16 GiB) shared by
-// `views` resident views.
-uint64_t HubLimit(vsgm_ctx* ctx, uint64_t option, size_t views) {
-  if (option != ~0ull) return option;
-  uint64_t budget = 0;
-  vsgm_mem_info(ctx, nullptr, &budget, nullptr);
-  return std::min<uint64_t>(budget / 8, 16ull << 30) / std::max<size_t>(views, 1);
+void PipelineExecutor::SetGraph(const Graph& g) {
+  VSGM_CHECK(g.vertex_count == vertex_count_, "graph vertex count does not match the executor");
+  ReleaseGraph();
+  graph_rp_ = std::make_shared<DeviceBuffer>(ctx_, (vertex_count_ + 1) * sizeof(uintE));
+  graph_cols_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(g.cols.size(), 1) * sizeof(uintV));
+  Check(vsgm_h2d(ctx_, graph_rp_->get(), g.row_ptrs.data(), (vertex_count_ + 1) * sizeof(uintE)), "row_ptrs upload");
+  Check(vsgm_h2d(ctx_, graph_cols_->get(), g.cols.data(), g.cols.size() * sizeof(uintV)), "cols upload");
+  Check(vsgm_ctx_sync(ctx_), "graph upload");
 }
-}  // namespace
+
+size_t PipelineExecutor::AddDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree) {
+  VSGM_CHECK(graph_rp_ && graph_cols_, "device view before SetGraph");
+  ReleaseGraph();
+  View v;
+  v.sources = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources.size(), 1) * sizeof(uintV));
+  v.sources_num = sources.size();
+  Check(vsgm_h2d(ctx_, v.sources->get(), sources.data(), sources.size() * sizeof(uintV)), "sources upload");
+  vsgm_view_out o;
+  Check(vsgm_view_build(ctx_, (const uint64_t*)graph_rp_->get(), (const uint32_t*)graph_cols_->get(), vertex_count_,
+                        (const uint32_t*)v.sources->get(), sources.size(), hop, root_degree, &o),
+        "view build");
+  v.row_ptrs = std::make_shared<DeviceBuffer>(ctx_, (void*)o.row_ptrs);
+  v.cols = std::make_shared<DeviceBuffer>(ctx_, (void*)o.cols);
+  std::vector<uint64_t> rp(vertex_count_ + 1);
+  Check(vsgm_d2h(ctx_, rp.data(), o.row_ptrs, rp.size() * sizeof(uint64_t)), "view row_ptrs readback");
+  IndexView(v, rp.data(), HubLimit(ctx_, opt_.hub_bytes, 8));
+  views_.push_back(std::move(v));
+  cur_ = views_.size() - 1;
+  return cur_;
+}
+
+void PipelineExecutor::ViewCsr(size_t view, std::vector<uint64_t>& row_ptrs, std::vector<uint32_t>& cols) const {
+  VSGM_CHECK(view < views_.size(), "view index out of range");
+  const View& v = views_[view];
+  row_ptrs.resize(vertex_count_ + 1);
+  Check(vsgm_d2h(ctx_, row_ptrs.data(), v.row_ptrs->get(), row_ptrs.size() * sizeof(uint64_t)), "view readback");
+  cols.resize(row_ptrs.back());
+  Check(vsgm_d2h(ctx_, cols.data(), v.cols->get(), cols.size() * sizeof(uint32_t)), "view readback");
+}
 
 void PipelineExecutor::Transfer(const ViewBinHolder& vbh) {
   VSGM_CHECK(vbh.GetVertexCount() == vertex_count_, "view vertex count does not match the executor");

@@ -70,6 +70,13 @@ class PipelineExecutor {
   // its index and makes it the current view.
   size_t AddView(const ViewBinHolder& vbh);
   size_t ViewCount() const { return views_.size(); }
+  // Keep the whole graph resident in HBM; AddDeviceView then builds a bin's
+  // view on the device from it (no host materialisation, only the sources
+  // cross PCIe) and keeps it resident like AddView.
+  void SetGraph(const Graph& g);
+  size_t AddDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree);
+  // Host copy of a resident view's CSR.
+  void ViewCsr(size_t view, std::vector<uint64_t>& row_ptrs, std::vector<uint32_t>& cols) const;
   // Enumerate every plan over the current view (pipeline_executor.h:83-109).
   void Match(bool do_match = true);
   // Enumerate the plans in `which` (all when empty) over every resident view.
@@ -104,6 +111,7 @@ class PipelineExecutor {
     vsgm_hub_index hub{nullptr, nullptr, 0, 0};
   };
   void ClearViews();
+  void IndexView(View& v, const uint64_t* h_row_ptrs, uint64_t hub_limit);
   View UploadView(const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n_cols, const uint32_t* sources, uint64_t n_sources,
                   uint64_t hub_limit);
   void MatchPlans(const std::vector<size_t>& views, const std::vector<int>& which);
@@ -122,6 +130,7 @@ class PipelineExecutor {
 
   std::vector<View> views_;
   size_t cur_ = 0;
+  std::shared_ptr<DeviceBuffer> graph_rp_, graph_cols_;  // whole graph (SetGraph)
   const View* view_ = nullptr;  // view being matched
 
   const Plan* plan_ = nullptr;  // plan being matched
