@@ -12,6 +12,22 @@
 
 namespace vsgm {
 
+namespace {
+// VSGM_PROGRESS=<seconds> (> 0): the generators report each build phase on
+// stderr, so that a minutes-long build of a billion-edge graph is visibly
+// making progress.
+void Phase(const char* what) {
+  static const bool on = [] {
+    const char* s = getenv("VSGM_PROGRESS");
+    return s && atof(s) > 0;
+  }();
+  if (!on) return;
+  static const double t0 = omp_get_wtime();
+  fprintf(stderr, "vsgm: graph build: %s (%.1f s)\n", what, omp_get_wtime() - t0);
+  fflush(stderr);
+}
+}  // namespace
+
 uint64_t Graph::MaxDegree() const {
   uint64_t m = 0;
   for (uint64_t v = 0; v < vertex_count; ++v) m = std::max<uint64_t>(m, row_ptrs[v + 1] - row_ptrs[v]);
@@ -98,20 +114,26 @@ Graph BuildFromEdges(uint64_t vertex_count, std::vector<std::pair<uintV, uintV>>
     deg[e.first].fetch_add(1, std::memory_order_relaxed);
     deg[e.second].fetch_add(1, std::memory_order_relaxed);
   }
+  Phase("degrees counted");
   std::vector<uint64_t> start(vertex_count + 1, 0);
   for (uint64_t v = 0; v < vertex_count; ++v) start[v + 1] = start[v] + deg[v].load(std::memory_order_relaxed);
   std::vector<uintV> raw(start[vertex_count]);
   std::vector<std::atomic<uint64_t>> fillp(vertex_count);
 #pragma omp parallel for num_threads(nt) schedule(static)
   for (uint64_t v = 0; v < vertex_count; ++v) fillp[v].store(start[v], std::memory_order_relaxed);
+  for (uint64_t s0 = 0; s0 < m; s0 += (m + 7) / 8) {
+    const uint64_t s1 = std::min(m, s0 + (m + 7) / 8);
 #pragma omp parallel for num_threads(nt) schedule(static)
-  for (uint64_t i = 0; i < m; ++i) {
-    auto e = edges[i];
-    if (e.first == e.second) continue;
-    raw[fillp[e.first].fetch_add(1, std::memory_order_relaxed)] = e.second;
-    raw[fillp[e.second].fetch_add(1, std::memory_order_relaxed)] = e.first;
+    for (uint64_t i = s0; i < s1; ++i) {
+      auto e = edges[i];
+      if (e.first == e.second) continue;
+      raw[fillp[e.first].fetch_add(1, std::memory_order_relaxed)] = e.second;
+      raw[fillp[e.second].fetch_add(1, std::memory_order_relaxed)] = e.first;
+    }
+    Phase("lists filling (slice)");
   }
   std::vector<std::pair<uintV, uintV>>().swap(edges);
+  Phase("lists filled");
   // sort + unique each list, then compact
   std::vector<uint64_t> kept(vertex_count + 1, 0);
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 1024)
@@ -120,11 +142,13 @@ Graph BuildFromEdges(uint64_t vertex_count, std::vector<std::pair<uintV, uintV>>
     std::sort(b, e);
     kept[v] = std::unique(b, e) - b;
   }
+  Phase("lists sorted");
   g.row_ptrs.assign(vertex_count + 1, 0);
   for (uint64_t v = 0; v < vertex_count; ++v) g.row_ptrs[v + 1] = g.row_ptrs[v] + kept[v];
   g.cols.resize(g.row_ptrs[vertex_count]);
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 1024)
   for (uint64_t v = 0; v < vertex_count; ++v) std::copy(raw.begin() + start[v], raw.begin() + start[v] + kept[v], g.cols.begin() + g.row_ptrs[v]);
+  Phase("CSR compacted");
   return g;
 }
 
@@ -174,14 +198,20 @@ Graph GenerateChungLu(uint64_t vertex_count, uint64_t draws, double gamma, uint6
   for (auto& x : cdf) x /= acc;
   cdf[vertex_count - 1] = 1.0;
   std::vector<std::pair<uintV, uintV>> edges(draws);
+  Phase("Chung-Lu weights");
+  // in eight slices, so that progress shows during a billion-draw build
+  for (uint64_t s0 = 0; s0 < draws; s0 += (draws + 7) / 8) {
+    const uint64_t s1 = std::min(draws, s0 + (draws + 7) / 8);
 #pragma omp parallel for num_threads(nt) schedule(static)
-  for (uint64_t i = 0; i < draws; ++i) {
-    double r0 = UniformAt(seed, 0, i), r1 = UniformAt(seed, 1, i);
-    uint64_t u = std::upper_bound(cdf.begin(), cdf.end(), r0) - cdf.begin();
-    uint64_t v = std::upper_bound(cdf.begin(), cdf.end(), r1) - cdf.begin();
-    if (u >= vertex_count) u = vertex_count - 1;
-    if (v >= vertex_count) v = vertex_count - 1;
-    edges[i] = std::make_pair((uintV)u, (uintV)v);
+    for (uint64_t i = s0; i < s1; ++i) {
+      double r0 = UniformAt(seed, 0, i), r1 = UniformAt(seed, 1, i);
+      uint64_t u = std::upper_bound(cdf.begin(), cdf.end(), r0) - cdf.begin();
+      uint64_t v = std::upper_bound(cdf.begin(), cdf.end(), r1) - cdf.begin();
+      if (u >= vertex_count) u = vertex_count - 1;
+      if (v >= vertex_count) v = vertex_count - 1;
+      edges[i] = std::make_pair((uintV)u, (uintV)v);
+    }
+    Phase("Chung-Lu edges drawn (slice)");
   }
   return BuildFromEdges(vertex_count, edges, nt);
 }
