@@ -5,7 +5,8 @@ import re
 
 from conftest import ROOT
 
-HEADERS = {"vsgm_hip.h": "libvsgm_hip.so", "vsgm_mlp.h": "libvsgm_hip.so", "vsgm.h": "libvsgm.so"}
+HEADERS = {"vsgm_hip.h": "libvsgm_hip.so", "vsgm_mlp.h": "libvsgm_hip.so", "vsgm_partition.h": "libvsgm_hip.so",
+           "vsgm.h": "libvsgm.so"}
 
 
 def _declared(header):

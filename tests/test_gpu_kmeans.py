@@ -1,0 +1,103 @@
+"""GPU: the k-means partitioner's kernels against the CPU checker
+(oracle/kmeans_ref.py) — one assignment pass, one centroid update, whole
+runs — and the view sizes of k-means bins against random bins.
+
+Float64 sums are accumulated in different orders on the two sides, so
+similarities and centroids agree to ~1e-12 relative; an assignment may
+differ only where two clusters are within that of each other."""
+import numpy as np
+import pytest
+
+from oracle import kmeans_ref as K
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(vsgm, gpu):
+    from vsgm_amd.partition import DeviceGraph
+
+    g = vsgm.Graph.rmat(8, 8, seed=3)
+    rp, cl = g.csr()
+    return g, rp, cl, DeviceGraph(g)
+
+
+def test_assign_pass(small):
+    from vsgm_amd.partition import assign_step
+
+    g, rp, cl, dg = small
+    X = K.features(rp, cl)
+    rng = np.random.default_rng(0)
+    C, _ = K.update(X, rng.integers(0, 5, g.vertex_count), 5)
+    norm = np.sqrt((C * C).sum(1))
+    a, s = assign_step(dg, C.T.copy(), norm)
+    S = K.similarities(X, C)
+    want_a, want_s = K.assign(X, C)
+    assert np.allclose(s, want_s, rtol=1e-12, atol=1e-14)
+    # the chosen cluster is a maximum; it is the reference's wherever the top two differ
+    assert np.all(S[np.arange(len(a)), a] >= want_s - 1e-12)
+    top2 = np.sort(S, axis=1)[:, -2:]
+    clear = top2[:, 1] - top2[:, 0] > 1e-9
+    assert np.array_equal(a[clear], want_a[clear])
+
+
+def test_update_pass(small):
+    from vsgm_amd.partition import update_step
+
+    g, rp, cl, dg = small
+    X = K.features(rp, cl)
+    a = np.random.default_rng(1).integers(0, 4, g.vertex_count)
+    a[:10] = 3
+    cent, norm, sizes = update_step(dg, a.astype(np.int32), 4)
+    C, want_sizes = K.update(X, a, 4)
+    assert np.array_equal(sizes, want_sizes.astype(np.uint64))
+    assert np.allclose(cent, C.T, rtol=1e-12, atol=1e-15)
+    assert np.allclose(norm, np.sqrt((C * C).sum(1)), rtol=1e-12)
+
+
+def test_whole_run_matches_checker(vsgm, small):
+    from vsgm_amd.partition import first_vertex, kmeans
+
+    g, rp, cl, _ = small
+    for k, seed in ((2, 1), (5, 7), (8, 3)):
+        a, sizes, obj = kmeans(g, k, iters=4, seed=seed)
+        want_a, want_sizes, want_obj = K.kmeans(rp, cl, k, 4, first=first_vertex(seed, g.vertex_count))
+        assert int(sizes.sum()) == g.vertex_count and np.array_equal(np.bincount(a, minlength=k), sizes)
+        assert np.allclose(obj, want_obj, rtol=1e-9), (k, obj, want_obj)
+        assert np.mean(a == want_a) > 0.99, k
+
+
+def test_disjoint_cliques(vsgm, gpu):
+    from vsgm_amd.partition import kmeans
+
+    m, k = 7, 5
+    g = vsgm.Graph.from_edges(m * k, [(c * m + i, c * m + j) for c in range(k) for i in range(m) for j in range(i + 1, m)])
+    a, sizes, obj = kmeans(g, k, iters=3, seed=11)
+    assert sorted(sizes.tolist()) == [m] * k
+    assert all(len(set(a[c * m:(c + 1) * m].tolist())) == 1 for c in range(k))
+
+
+def test_kmeans_bins_make_smaller_views(vsgm, gpu):
+    """The point of the partition: 2-hop views of k-means bins hold fewer
+    list entries than views of random bins of the same sizes."""
+    from vsgm_amd.partition import kmeans
+
+    g = vsgm.Graph.rmat(14, 16, seed=2)
+    k = 8
+    a, sizes, obj = kmeans(g, k, iters=10, seed=1)
+    assert all(y >= x - 1e-6 * abs(x) for x, y in zip(obj, obj[1:]))
+    rand = np.random.default_rng(0).permutation(np.repeat(np.arange(k), sizes.astype(np.int64)))
+
+    def view_entries(assign):
+        vbm = vsgm.ViewBinManager.from_assignment(g, 2, k, assign.astype(np.int32))
+        h = vsgm.ViewBinHolder(g.vertex_count, g.entries * 4 + 4)
+        tot = 0
+        for b in range(vbm.n_bins):
+            vbm.materialize(b, h, 0)
+            tot += h.info()["entries"]
+        return tot
+
+    km, rnd = view_entries(a), view_entries(rand)
+    print(f"[gpu] R-MAT s14, k=8: view entries k-means {km} vs random {rnd} ({km / rnd:.3f}); objective {obj[0]:.1f} -> {obj[-1]:.1f}",
+          flush=True)
+    assert km < rnd

@@ -1,0 +1,64 @@
+"""CPU: pinning the k-means checker (oracle/kmeans_ref.py) — cosine
+similarity against the set formula, centroid sums against their closed
+form, exact recovery of disjoint cliques, and a non-decreasing objective."""
+import numpy as np
+
+from oracle import kmeans_ref as K
+
+
+def csr(n, edges):
+    adj = [set() for _ in range(n)]
+    for a, b in edges:
+        if a != b:
+            adj[a].add(b)
+            adj[b].add(a)
+    rp = np.zeros(n + 1, np.uint64)
+    rp[1:] = np.cumsum([len(x) for x in adj])
+    cols = np.array([y for x in adj for y in sorted(x)], np.uint32)
+    return rp, cols, adj
+
+
+def test_similarity_is_the_set_cosine():
+    rng = np.random.default_rng(0)
+    n = 25
+    rp, cols, adj = csr(n, [tuple(rng.integers(0, n, 2)) for _ in range(60)])
+    X = K.features(rp, cols)
+    for u in (0, 3, 11):
+        s = K.similarities(X, X[u:u + 1])[:, 0]
+        Nu = adj[u] | {u}
+        for v in range(n):
+            Nv = adj[v] | {v}
+            assert abs(s[v] - len(Nu & Nv) / np.sqrt(len(Nu) * len(Nv))) < 1e-12
+
+
+def test_centroid_sums_closed_form():
+    rng = np.random.default_rng(1)
+    n = 30
+    rp, cols, adj = csr(n, [tuple(rng.integers(0, n, 2)) for _ in range(80)])
+    X = K.features(rp, cols)
+    a = rng.integers(0, 3, n)
+    C, sizes = K.update(X, a, 3)
+    for k in range(3):
+        members = np.nonzero(a == k)[0]
+        # each feature sums to sqrt(|N[v]|): the centroid sums to their mean
+        want = np.mean([np.sqrt(len(adj[v]) + 1) for v in members])
+        assert abs(C[k].sum() - want) < 1e-12 and sizes[k] == len(members)
+
+
+def test_disjoint_cliques_are_recovered():
+    m, k = 6, 4
+    edges = [(c * m + i, c * m + j) for c in range(k) for i in range(m) for j in range(i + 1, m)]
+    rp, cols, _ = csr(m * k, edges)
+    a, sizes, obj = K.kmeans(rp, cols, k, 3, first=7)
+    assert sorted(sizes.tolist()) == [m] * k
+    for c in range(k):
+        assert len(set(a[c * m:(c + 1) * m].tolist())) == 1  # one cluster per clique
+    assert abs(obj[-1] - m * k) < 1e-9  # every vertex identical to its centroid
+
+
+def test_objective_does_not_decrease():
+    rng = np.random.default_rng(2)
+    n = 80
+    rp, cols, _ = csr(n, [tuple(rng.integers(0, n, 2)) for _ in range(300)])
+    _, _, obj = K.kmeans(rp, cols, 4, 6, first=5)
+    assert all(b >= a - 1e-9 for a, b in zip(obj, obj[1:]))
