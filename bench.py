@@ -337,11 +337,14 @@ def calibration_main():
 
 
 def batched_main(args):
-    """Many patterns over k resident view bins (the batched configuration):
+    """Many patterns over k view bins from a GPU k-means partition (the
+    batched configuration):
       plain    one pattern at a time, one bin at a time, heuristic plans
-               (transfer each bin, then Match one plan after another);
-      batched  all bins resident, every plan autotuned, the whole
-               enumeration captured once into one device graph and replayed;
+               (host-built view of each bin transferred, then Match one plan
+               after another);
+      batched  all bins' views built on the device and resident, every plan
+               autotuned, the whole enumeration captured once into one
+               device graph and replayed;
       learned  as batched, with matching orders from the learned planner
                (trained here on records from smaller seeded R-MAT graphs).
     Every count is checked equal across the three. Times are the match phase
@@ -376,10 +379,17 @@ def batched_main(args):
     model, losses = learned.train(data, epochs=60, batch=128, lr=2e-3, seed=args.seed)
     log(f"learned planner: {len(data['y'])} records, loss {losses[0]:.3f} -> {losses[-1]:.3f} in {time.perf_counter() - t:.1f}s")
 
+    # the bins: k-means on the GPU over closed 1-hop neighbourhoods
+    from vsgm_amd import partition
+
+    t = time.perf_counter()
+    assign, km_sizes, km_obj = partition.kmeans(g, args.bins, iters=10, seed=args.seed)
+    log(f"k-means k={args.bins}: sizes {km_sizes.tolist()}, objective {km_obj[0]:.1f} -> {km_obj[-1]:.1f} "
+        f"in {time.perf_counter() - t:.2f}s")
+
     def make_bins(plans):
         hop = max(p.hop for p in plans)
         rd = min(p.root_degree for p in plans)
-        assign = np.random.default_rng(args.seed + 7).integers(0, args.bins, size=g.vertex_count).astype(np.int32)
         vbm = V.ViewBinManager.from_assignment(g, hop, args.bins, assign)
         holders = []
         for b in range(vbm.n_bins):
@@ -413,17 +423,31 @@ def batched_main(args):
     plain_counts = ex.total_counts
     del ex
 
+    # batched modes build the same bins' views on the device from the resident graph
+    all_plans = heur + lplans
+    v_hop = max(p.hop for p in all_plans)
+    v_rd = min(p.root_degree for p in all_plans)
+    bin_sources = [h.csr(g.vertex_count)[2] for h in holders]
+
     def batched(plans, label):
         # tune on the first bin alone (a sample of the roots), then keep every bin resident
+        # with every bin resident and one graph for all plans, a fixed 2 GiB
+        # level budget (0 in the tuning below) keeps the pool from fragmenting
         tx = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
-        tx.add_view(holders[0])
+        tx.set_option("batch_bytes", 2 << 30)
+        tx.set_graph(g)
+        tx.add_device_view(bin_sources[0], v_hop, v_rd)
         t = time.perf_counter()
         res = autotune.tune(tx, repeats=1, candidates={"isect_kernel": [0, 2], "max_grid": [8192], "batch_bytes": [0, 1 << 30]})
         tune_s = time.perf_counter() - t
         del tx
         bx = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
-        for h in holders:
-            bx.add_view(h)
+        bx.set_option("batch_bytes", 2 << 30)
+        bx.set_graph(g)
+        t = time.perf_counter()
+        for s in bin_sources:
+            bx.add_device_view(s, v_hop, v_rd)
+        log(f"{label}: {len(bin_sources)} device views in {time.perf_counter() - t:.3f}s")
         for i, r in res.items():
             bx.set_plan_tuning(i, **r["config"])
         t = time.perf_counter()
@@ -458,7 +482,7 @@ def batched_main(args):
         "data": f"synthetic: seeded R-MAT (seed {args.seed}) generated on the host",
         "config": {
             "workload": f"batched: {len(queries)} random connected 3-6-vertex patterns (seed {args.seed}) over "
-                        f"{vbm.n_bins} view bins of R-MAT scale {args.scale} ef {args.edge_factor}",
+                        f"{vbm.n_bins} k-means view bins ({v_hop}-hop) of R-MAT scale {args.scale} ef {args.edge_factor}",
             "scale": args.scale, "bins": vbm.n_bins, "patterns": len(queries),
             "total_embeddings": total,
             "plain_s": plain_s, "batched_s": batched_s, "learned_batched_s": learned_s,

@@ -1989,6 +1989,13 @@ int vsgm_capture_begin(vsgm_ctx* ctx) {
     HIP_TRY(ctx, hipGraphExecDestroy(ctx->graph_exec));
     ctx->graph_exec = nullptr;
   }
+  // Blocks cached before the capture are referenced by no captured node:
+  // return them now, since during the capture (and while its graph lives)
+  // the pool cannot hand cached memory back to recover from a failed
+  // allocation.
+  for (auto& kv : ctx->cached) (void)hipFree(kv.second);
+  ctx->cached.clear();
+  ctx->cached_bytes = 0;
   ctx->saved_profiling = ctx->profiling;
   ctx->profiling = false;
   ctx->tape_mode = 2;
