@@ -106,6 +106,9 @@ def main():
                     help="single: the 4-clique/diamond/4-cycle line; batched: many random patterns over k view bins, "
                          "one-at-a-time vs batched+tuned vs learned+batched; copy-calibration: kernels of known HBM "
                          "traffic (a 2 GiB fill and a 2 GiB reduction) that scale the rocprof byte counters")
+    ap.add_argument("--plan", choices=["heuristic", "learned"], default="heuristic",
+                    help="matching orders of the single workload: the heuristic planner (the metric's configuration) "
+                         "or the learned planner")
     ap.add_argument("--n-patterns", type=int, default=64)
     ap.add_argument("--bins", type=int, default=8)
     args = ap.parse_args()
@@ -137,6 +140,39 @@ def main():
 
     queries = [make_query(V, k) for k in keys]
     plans = [V.Plan.optimize(q) for q in queries]
+    learned_info = None
+    if args.plan == "learned":
+        # the learned planner: records from small seeded R-MAT graphs, HIP-trained
+        # cost model, order per pattern (heuristic plan kept where no order is
+        # predicted to gain)
+        import numpy as np
+
+        from vsgm_amd import learned
+
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        from shapes import all_shapes
+
+        t = time.perf_counter()
+        tq = [V.Query.from_matrix(s[2]) for s in all_shapes((3, 4, 5))]
+        data = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (10, 11)], tq, bins=2, max_orders=24, repeats=1,
+                               seed=args.seed)
+        # 4-vertex shapes also on larger graphs, where order costs separate
+        # from launch overheads (scale 11 times are ~1 ms)
+        q4 = [V.Query.from_matrix(s[2]) for s in all_shapes((4,))]
+        big = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (13, 14)], q4, bins=2, max_orders=24, repeats=1,
+                              seed=args.seed + 1)
+        big["group"] = big["group"] + int(data["group"].max()) + 1
+        data = {k: np.concatenate([data[k], big[k]]) for k in data}
+        model, losses = learned.train(data, epochs=60, batch=128, lr=2e-3, seed=args.seed)
+        rp_all, _ = g.csr(copy=False)
+        vstats = learned.view_stats(rp_all, g.vertex_count)
+        planner = learned.LearnedPlanner(model, margin=0.1)
+        chosen = [planner.plan(q, vstats) for q in queries]
+        plans = [p for p, _ in chosen]
+        learned_info = {"records": int(len(data["y"])), "loss": [float(losses[0]), float(losses[-1])],
+                        "orders": {k: (info.get("order") if info["learned"] else "heuristic") for k, (_, info) in zip(keys, chosen)},
+                        "train_s": round(time.perf_counter() - t, 1)}
+        log(f"learned planner: {learned_info}")
     hop = max(p.hop for p in plans)
     root_degree = min(p.root_degree for p in plans)
 
@@ -228,14 +264,15 @@ def main():
         "data": f"synthetic: seeded R-MAT (seed {args.seed}) generated on the host",
         "config": {
             "workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (.57/.19/.19/.05), single-bin {hop}-hop view, "
-                        f"patterns {','.join(keys)}, heuristic plans",
+                        f"patterns {','.join(keys)}, {args.plan} plans",
             "scale": args.scale,
             "vertices": g.vertex_count,
             "edges": g.edge_count,
             "patterns": keys,
             "counts": counts,
             "per_pattern": per_pattern,
-            "plan": "heuristic",
+            "plan": args.plan,
+            "learned_planner": learned_info,
             "graph_gen_s": round(gen_s, 2),
             "view_build_s": round(view_s, 2),
             "transfer_s": round(xfer_s, 3),
