@@ -171,6 +171,8 @@ def main():
         plans = [p for p, _ in chosen]
         learned_info = {"records": int(len(data["y"])), "loss": [float(losses[0]), float(losses[-1])],
                         "orders": {k: (info.get("order") if info["learned"] else "heuristic") for k, (_, info) in zip(keys, chosen)},
+                        "predicted_gain": {k: (round(info["pred_gain"], 4), info.get("best_order", info.get("order")))
+                                           for k, (_, info) in zip(keys, chosen)},
                         "train_s": round(time.perf_counter() - t, 1)}
         log(f"learned planner: {learned_info}")
     hop = max(p.hop for p in plans)

@@ -10,8 +10,13 @@ to the heuristic order's time on the same view. The label is
 log(time(order) / time(heuristic order)); the model learns relative cost, so
 patterns and views of very different sizes share one scale.
 
-Features (IN_DIM = 110, all in [0, 1] or small):
-  64  the pattern's adjacency permuted into the matching order (8 x 8, zero padded)
+Features (IN_DIM = 102, all in [-1, 1] or small):
+  28  the pattern's adjacency permuted into the matching order (upper
+      triangle of the 8 x 8 matrix, zero padded)
+  28  the symmetry-breaking conditions permuted the same way: +1 where
+      position i must take a smaller id than position j, -1 for larger, 0 for
+      none (orders that are images of each other under an automorphism
+      differ only here, and their costs differ)
    8  backward-neighbour count of each position / 7
   32  the view's degree histogram: fraction of view vertices with degree in
       [2^b, 2^(b+1)) for b = 0..31 (isolated vertices excluded)
@@ -33,8 +38,10 @@ import numpy as np
 from . import api as V
 from .mlp import CostModel
 
-IN_DIM = 110
+IN_DIM = 102
 MAXQ = 8
+_UPPER = np.triu_indices(MAXQ, 1)
+LESS_THAN, LARGER_THAN = 0, 1  # condition codes of Query.info()
 
 
 def connected_orders(adj: np.ndarray, limit: Optional[int] = None, seed: int = 0) -> List[Tuple[int, ...]]:
@@ -61,15 +68,18 @@ def view_stats(row_ptrs: np.ndarray, n_sources: int) -> np.ndarray:
     return np.concatenate([hist, sizes]).astype(np.float32)
 
 
-def featurize(adj: np.ndarray, order: Sequence[int], hop: int, vstats: np.ndarray) -> np.ndarray:
+def featurize(adj: np.ndarray, cond: np.ndarray, order: Sequence[int], hop: int, vstats: np.ndarray) -> np.ndarray:
     n = adj.shape[0]
-    perm = np.zeros((MAXQ, MAXQ), np.float32)
     o = list(order)
+    perm = np.zeros((MAXQ, MAXQ), np.float32)
     perm[:n, :n] = adj[np.ix_(o, o)]
+    pc = np.zeros((MAXQ, MAXQ), np.float32)
+    c = cond[np.ix_(o, o)]
+    pc[:n, :n] = np.where(c == LESS_THAN, 1.0, np.where(c == LARGER_THAN, -1.0, 0.0))
     back = np.zeros(MAXQ, np.float32)
     for i in range(1, n):
         back[i] = perm[i, :i].sum() / 7.0
-    return np.concatenate([perm.ravel(), back, vstats, [n / 8.0, hop / 4.0]]).astype(np.float32)
+    return np.concatenate([perm[_UPPER], pc[_UPPER], back, vstats, [n / 8.0, hop / 4.0]]).astype(np.float32)
 
 
 def _time_plans(ex, repeats: int) -> List[float]:
@@ -94,7 +104,7 @@ def collect(graphs, queries: Sequence["V.Query"], bins: int = 2, max_orders: int
     for gi, g in enumerate(graphs):
         assign = rng.integers(0, bins, size=g.vertex_count).astype(np.int32)
         for q in queries:
-            n, _, adj, _ = q.info()
+            n, _, adj, cond = q.info()
             heur = V.Plan.optimize(q)
             orders = connected_orders(adj, max_orders, seed=seed + gid)
             if tuple(heur.order) not in orders:
@@ -118,7 +128,7 @@ def collect(graphs, queries: Sequence["V.Query"], bins: int = 2, max_orders: int
                 vs = view_stats(rp, len(src))
                 h = orders.index(tuple(heur.order))
                 for o, p, t in zip(orders, plans, ms):
-                    xs.append(featurize(adj, o, p.hop, vs))
+                    xs.append(featurize(adj, cond, o, p.hop, vs))
                     ys.append(math.log((t + 1e-3) / (ms[h] + 1e-3)))
                     ms_all.append(t)
                     groups.append(gid)
@@ -146,18 +156,18 @@ class LearnedPlanner:
         self.model, self.max_orders, self.margin = model, max_orders, margin
 
     def plan(self, query: "V.Query", vstats: np.ndarray):
-        n, _, adj, _ = query.info()
+        n, _, adj, cond = query.info()
         heur = V.Plan.optimize(query)
         orders = connected_orders(adj, self.max_orders)
         h = tuple(heur.order)
         if h not in orders:
             orders.append(h)
         hops = [heur.hop if o == h else V.Plan.from_order(query, o).hop for o in orders]
-        x = np.stack([featurize(adj, o, hp, vstats) for o, hp in zip(orders, hops)])
+        x = np.stack([featurize(adj, cond, o, hp, vstats) for o, hp in zip(orders, hops)])
         pred = self.model.forward(x)
         best = int(np.argmin(pred))
         hi = orders.index(h)
         if orders[best] == h or pred[best] > pred[hi] - self.margin:
-            return heur, {"learned": False, "pred_gain": float(pred[hi] - pred[best])}
+            return heur, {"learned": False, "pred_gain": float(pred[hi] - pred[best]), "best_order": list(orders[best])}
         return V.Plan.from_order(query, orders[best]), {"learned": True, "pred_gain": float(pred[hi] - pred[best]),
                                                         "order": list(orders[best])}
