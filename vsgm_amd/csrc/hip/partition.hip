@@ -117,6 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_km_scatter(const uint64_t* __restric
   const int lane = threadIdx.x & (kWave - 1);
   for (uint64_t v = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < V; v += (uint64_t)gridDim.x * kWavesPerBlock) {
     const int k = assign[v];
+    if (k < 0 || k >= K) continue;  // a caller's assignment out of range joins no cluster
     const double r = sqrt(1.0 / (double)(rp[v + 1] - rp[v] + 1));
     for (uint64_t j = rp[v] + lane; j < rp[v + 1]; j += kWave) unsafeAtomicAdd(&cent[(uint64_t)cols[j] * K + k], r);
     if (lane == 0) unsafeAtomicAdd(&cent[v * K + k], r);
@@ -127,7 +128,10 @@ __global__ __launch_bounds__(kBlock) void k_km_sizes(const int32_t* __restrict__
   __shared__ unsigned long long s_hist[VSGM_KMEANS_MAX_K];
   if (threadIdx.x < VSGM_KMEANS_MAX_K) s_hist[threadIdx.x] = 0;
   __syncthreads();
-  for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) atomicAdd(&s_hist[assign[v]], 1ull);
+  for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+    const int k = assign[v];
+    if (k >= 0 && k < K) atomicAdd(&s_hist[k], 1ull);
+  }
   __syncthreads();
   if (threadIdx.x < (unsigned)K && s_hist[threadIdx.x]) atomicAdd(&sizes[threadIdx.x], s_hist[threadIdx.x]);
 }

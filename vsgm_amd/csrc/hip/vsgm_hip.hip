@@ -511,9 +511,10 @@ __global__ void k_copy1(const uint64_t* __restrict__ src, uint64_t* __restrict__
 constexpr uint8_t kFar = 0xFF;
 
 __global__ __launch_bounds__(kBlock) void k_view_seed(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ sources,
-                                                      uint64_t n_sources, uint64_t root_degree, uint8_t* __restrict__ dist) {
+                                                      uint64_t n_sources, uint64_t V, uint64_t root_degree, uint8_t* __restrict__ dist) {
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n_sources; i += (uint64_t)gridDim.x * kBlock) {
     const uint32_t s = sources[i];
+    if (s >= V) continue;  // not a vertex of this graph
     if (root_degree == 0 || row_ptrs[s + 1] - row_ptrs[s] >= root_degree) dist[s] = 0;
   }
 }
@@ -1876,7 +1877,7 @@ int vsgm_ctx_create(int device, uint64_t mem_budget, vsgm_ctx** out) {
   vsgm_ctx* ctx = new vsgm_ctx();
   ctx->device = device;
   auto bail = [&](int code) {
-    delete ctx;
+    vsgm_ctx_destroy(ctx);  // releases whatever was created so far
     return code;
   };
   if (hipSetDevice(device) != hipSuccess) return bail(VSGM_ERR_HIP);
@@ -2203,7 +2204,7 @@ int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d
   if ((s = launch_check(ctx, "k_fill")) != VSGM_OK) return undo(s);
   if (n_sources) {
     hipLaunchKernelGGL(k_view_seed, dim3(grid_for((n_sources + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_sources,
-                       n_sources, root_degree, dist);
+                       n_sources, V, root_degree, dist);
     if ((s = launch_check(ctx, "k_view_seed")) != VSGM_OK) return undo(s);
   }
   const unsigned wave_grid = grid_for((V + kBlock / kWave - 1) / (kBlock / kWave));
