@@ -216,3 +216,20 @@ def test_pm_pipeline_cli(vsgm, oracle, gpu, tmp_path):
         assert out.returncode == 0, out.stderr[-2000:]
         line = [x for x in out.stdout.splitlines() if x.startswith("total count for query 0:")][0]
         assert int(line.split(":")[1]) == oracle_counts(oracle, g, [vsgm.Plan.optimize(q)])[0]
+
+
+def test_pm_pipeline_cli_device_views(vsgm, oracle, gpu, tmp_path):
+    """-gk partitions with the GPU k-means instead of the .vbmap file; -dv 1
+    builds every view on the consumer's GPU. Counts stay those of the graph."""
+    g = vsgm.Graph.rmat(10, 16, seed=13)
+    path = str(tmp_path / "rmat10.bin")
+    g.write_bin(path)
+    exe = os.path.join(ROOT, "vsgm_amd", "bin", "pm_pipeline")
+    want = oracle_counts(oracle, g, [vsgm.Plan.optimize(vsgm.Query.clique(4))])[0]
+    for extra in (["-gk", "3"], ["-gk", "3", "-dv", "1"], ["-gk", "1", "-dv", "1"]):
+        print("[gpu] pm_pipeline", extra, flush=True)
+        out = subprocess.run([exe, "-f", path, "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4", "-kc", "4"] + extra,
+                             capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr[-2000:]
+        line = [x for x in out.stdout.splitlines() if x.startswith("total count for query 0:")][0]
+        assert int(line.split(":")[1]) == want

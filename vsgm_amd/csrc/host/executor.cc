@@ -210,8 +210,22 @@ void PipelineExecutor::SetGraph(const Graph& g) {
 }
 
 size_t PipelineExecutor::AddDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree) {
-  VSGM_CHECK(graph_rp_ && graph_cols_, "device view before SetGraph");
   ReleaseGraph();
+  // the automatic index size assumes up to 8 resident views
+  views_.push_back(BuildDeviceView(sources, hop, root_degree, HubLimit(ctx_, opt_.hub_bytes, 8)));
+  cur_ = views_.size() - 1;
+  return cur_;
+}
+
+void PipelineExecutor::TransferDevice(const std::vector<uintV>& sources, int hop, uint64_t root_degree) {
+  ClearViews();
+  views_.push_back(BuildDeviceView(sources, hop, root_degree, HubLimit(ctx_, opt_.hub_bytes, 1)));
+  cur_ = 0;
+}
+
+PipelineExecutor::View PipelineExecutor::BuildDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree,
+                                                         uint64_t hub_limit) {
+  VSGM_CHECK(graph_rp_ && graph_cols_, "device view before SetGraph");
   View v;
   v.sources = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources.size(), 1) * sizeof(uintV));
   v.sources_num = sources.size();
@@ -224,10 +238,8 @@ size_t PipelineExecutor::AddDeviceView(const std::vector<uintV>& sources, int ho
   v.cols = std::make_shared<DeviceBuffer>(ctx_, (void*)o.cols);
   std::vector<uint64_t> rp(vertex_count_ + 1);
   Check(vsgm_d2h(ctx_, rp.data(), o.row_ptrs, rp.size() * sizeof(uint64_t)), "view row_ptrs readback");
-  IndexView(v, rp.data(), HubLimit(ctx_, opt_.hub_bytes, 8));
-  views_.push_back(std::move(v));
-  cur_ = views_.size() - 1;
-  return cur_;
+  IndexView(v, rp.data(), hub_limit);
+  return v;
 }
 
 void PipelineExecutor::ViewCsr(size_t view, std::vector<uint64_t>& row_ptrs, std::vector<uint32_t>& cols) const {

@@ -75,6 +75,9 @@ class PipelineExecutor {
   // cross PCIe) and keeps it resident like AddView.
   void SetGraph(const Graph& g);
   size_t AddDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree);
+  // Build one bin's view on the device like AddDeviceView, replacing every
+  // resident view (the device-side counterpart of Transfer).
+  void TransferDevice(const std::vector<uintV>& sources, int hop, uint64_t root_degree);
   // Host copy of a resident view's CSR.
   void ViewCsr(size_t view, std::vector<uint64_t>& row_ptrs, std::vector<uint32_t>& cols) const;
   // Enumerate every plan over the current view (pipeline_executor.h:83-109).
@@ -112,6 +115,7 @@ class PipelineExecutor {
   };
   void ClearViews();
   void IndexView(View& v, const uint64_t* h_row_ptrs, uint64_t hub_limit);
+  View BuildDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree, uint64_t hub_limit);
   View UploadView(const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n_cols, const uint32_t* sources, uint64_t n_sources,
                   uint64_t hub_limit);
   void MatchPlans(const std::vector<size_t>& views, const std::vector<int>& which);

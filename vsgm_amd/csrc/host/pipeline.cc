@@ -2,6 +2,7 @@
 // producers materialise view bins into a ring of pinned holders, one
 // consumer per GPU transfers each bin and matches every plan, and the
 // per-consumer counts are summed at the end. Same flags and output lines.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -22,6 +23,7 @@
 #include "query.h"
 #include "view.h"
 #include "vsgm.h"
+#include "vsgm_partition.h"
 
 namespace vsgm {
 namespace {
@@ -78,6 +80,30 @@ double Ms(std::chrono::steady_clock::time_point a) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
 }
 
+// Vertex partition by the GPU k-means (include/vsgm_partition.h) on device 0.
+std::vector<int> GpuKMeans(const Graph& g, int k, int iters, uint64_t seed) {
+  vsgm_ctx* ctx = nullptr;
+  if (vsgm_ctx_create(0, 0, &ctx) != VSGM_OK) throw Error("GPU k-means needs a GPU");
+  void *rp = nullptr, *cols = nullptr;
+  std::vector<int32_t> assign(g.vertex_count);
+  std::vector<uint64_t> sizes(k);
+  auto check = [&](int s, const char* what) {
+    if (s != VSGM_OK) {
+      std::string msg = std::string(what) + " failed: " + vsgm_last_error(ctx);
+      vsgm_ctx_destroy(ctx);
+      throw Error(msg);
+    }
+  };
+  check(vsgm_alloc(ctx, (g.vertex_count + 1) * sizeof(uint64_t), &rp), "k-means allocation");
+  check(vsgm_alloc(ctx, std::max<size_t>(g.cols.size(), 1) * sizeof(uint32_t), &cols), "k-means allocation");
+  check(vsgm_h2d(ctx, rp, g.row_ptrs.data(), (g.vertex_count + 1) * sizeof(uint64_t)), "graph upload");
+  check(vsgm_h2d(ctx, cols, g.cols.data(), g.cols.size() * sizeof(uint32_t)), "graph upload");
+  check(vsgm_kmeans(ctx, (const uint64_t*)rp, (const uint32_t*)cols, g.vertex_count, k, iters, seed, assign.data(), sizes.data(), nullptr),
+        "k-means");
+  vsgm_ctx_destroy(ctx);  // frees the pool
+  return std::vector<int>(assign.begin(), assign.end());
+}
+
 int Run(int argc, char** argv) {
   Flags fl{argc, argv};
   const std::string filename = fl.str("-f", "./data/com-friendster.ungraph.txt.bin");
@@ -94,6 +120,14 @@ int Run(int argc, char** argv) {
   const int do_reorder = fl.i("-dr", 0);
   const int do_split = fl.i("-ds", 0);
   const int split_times = fl.i("-dst", 1);
+  // Options of this build: -gk K partitions the vertices with the GPU k-means
+  // (K clusters, -gki iterations, -gks seed) instead of reading the .vbmap
+  // file; -dv 1 builds each bin's view on the consumer's GPU from the whole
+  // graph kept resident there, instead of materialising it on the host.
+  const int gpu_k = fl.i("-gk", 0);
+  const int gpu_k_iters = fl.i("-gki", 10);
+  const int gpu_k_seed = fl.i("-gks", 1);
+  const int device_views = fl.i("-dv", 0);
 
   int gpus = 0;
   vsgm_device_count(&gpus);
@@ -129,14 +163,24 @@ int Run(int argc, char** argv) {
   std::cout << "read graph " << filename << " V=" << graph.vertex_count << " entries=" << graph.cols.size() << " in " << Ms(t0) << " ms"
             << std::endl;
   ViewBinManager vbm(&graph, hop, thread_num);
-  vbm.LoadViewBinPartition(ViewBinPartitionName(filename, hop));
+  if (gpu_k > 0) {
+    t0 = std::chrono::steady_clock::now();
+    std::vector<int> assignment = GpuKMeans(graph, gpu_k, gpu_k_iters, (uint64_t)gpu_k_seed);
+    vbm.SetPartition(gpu_k, assignment);
+    std::cout << "GPU k-means: " << gpu_k << " clusters in " << Ms(t0) << " ms" << std::endl;
+  } else {
+    vbm.LoadViewBinPartition(ViewBinPartitionName(filename, hop));
+  }
   if (do_reorder) vbm.Reorder();
   if (do_split) vbm.Split(consumers, split_times);
   const uint64_t max_sources = vbm.GetMaxPartitionedSourcesNum();
   const uint64_t max_view_bytes = (uint64_t)(mem * 1000 * 1000 * 1000);
 
+  // With -dv 1 there are no producers and no host holders: the consumers
+  // take bins in turn and build each view on their own GPU.
   std::vector<std::unique_ptr<ViewBinHolder>> holders;
-  for (int i = 0; i < queue_size; ++i) holders.emplace_back(new ViewBinHolder(graph.vertex_count, max_view_bytes));
+  if (!device_views)
+    for (int i = 0; i < queue_size; ++i) holders.emplace_back(new ViewBinHolder(graph.vertex_count, max_view_bytes));
   BoundedQueue assigned(queue_size), released(queue_size);
   for (int i = 0; i < queue_size; ++i) released.push(i);
 
@@ -150,7 +194,7 @@ int Run(int argc, char** argv) {
   };
   std::vector<std::thread> threads;
   const auto& pool = vbm.GetViewBinPool();
-  for (int p = 0; p < producers; ++p)
+  for (int p = 0; p < (device_views ? 0 : producers); ++p)
     threads.emplace_back([&, p] {
       try {
         for (int bin; (bin = next_bin++) < (int)pool.size() && !failed;) {
@@ -177,6 +221,23 @@ int Run(int argc, char** argv) {
     threads.emplace_back([&, c] {
       try {
         PipelineExecutor exec(c, graph.vertex_count, plan_ptrs, max_sources);
+        if (device_views) {
+          exec.SetGraph(graph);
+          for (int bin; (bin = next_bin++) < (int)pool.size() && !failed;) {
+            auto t = std::chrono::steady_clock::now();
+            exec.TransferDevice(pool[bin]->GetSources(), hop, root_degree);
+            std::ostringstream os;
+            os << "consumer " << c << " builds view " << bin << " (" << Ms(t) << " ms)\n";
+            say(os.str());
+            t = std::chrono::steady_clock::now();
+            exec.Match(do_match != 0);
+            std::ostringstream os2;
+            os2 << "consumer " << c << " completes match of view " << bin << " (" << Ms(t) << " ms)\n";
+            say(os2.str());
+          }
+          counts[c] = exec.GetTotalCounts();
+          return;
+        }
         int holder;
         while (assigned.pop(holder)) {
           auto t = std::chrono::steady_clock::now();
