@@ -40,6 +40,18 @@ int vsgm_graph_write_bin(const vsgm_graph* g, const char* path);
 int vsgm_graph_from_csr(uint64_t vertex_count, const uint64_t* row_ptrs, const uint32_t* cols, vsgm_graph** out);
 /* Graph from undirected edge pairs (cleaned: symmetrised, deduplicated, no self loops). */
 int vsgm_graph_from_edges(uint64_t vertex_count, const uint32_t* pairs, uint64_t n_pairs, vsgm_graph** out);
+/* SNAP edge-list text -> preprocessed undirected graph (GraphIO::ReadSNAPFile,
+ * src/graph/graph_io.h:193-274, then PreprocessGraph::Preprocess,
+ * src/preprocess/preprocess_graph.h:12-92): lines that do not start with a
+ * digit are skipped, ids are shifted so the smallest is 0, self loops,
+ * parallel edges and isolated vertices are dropped and the rest renumbered
+ * in order. device >= 0 builds the CSR on that GPU (vsgm_csr_from_pairs),
+ * device < 0 on the host; threads <= 0 uses all cores. directed != 0 is
+ * rejected (the matcher reads undirected graphs). */
+int vsgm_graph_read_snap(const char* path, int directed, int device, int threads, vsgm_graph** out);
+/* Copy of g without its isolated vertices, ids renumbered in order
+ * (preprocess_graph.h:67-82). */
+int vsgm_graph_drop_isolated(const vsgm_graph* g, vsgm_graph** out);
 int vsgm_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads, vsgm_graph** out);
 int vsgm_graph_chung_lu(uint64_t vertex_count, uint64_t draws, double gamma, uint64_t seed, int threads, vsgm_graph** out);
 int vsgm_graph_info(const vsgm_graph* g, uint64_t* vertex_count, uint64_t* entries, uint64_t* max_degree);
@@ -157,6 +169,11 @@ void vsgm_executor_free(vsgm_executor* e);
 
 /* ---- the pm_pipeline driver (src/app/pattern_matching_pipeline.cu:27-201) */
 int vsgm_pm_pipeline(int argc, char** argv);
+
+/* ---- the preprocess_graph tool (src/preprocess/preprocess_graph.cc:9-36):
+ * -f SNAP file, -d directed (0 only), -o 1 (.bin), plus -g GPU (>= 0 builds
+ * the CSR there) and -t threads; writes <file>.bin */
+int vsgm_preprocess_graph(int argc, char** argv);
 
 #ifdef __cplusplus
 }

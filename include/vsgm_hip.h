@@ -171,6 +171,20 @@ typedef struct vsgm_view_out {
 int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* d_sources,
                     uint64_t n_sources, int hop, uint64_t root_degree, vsgm_view_out* out);
 
+/* ---- graph ingestion ------------------------------------------------------
+ * Replaces GraphIO::ReadSNAPFile's CSR build + SortCSRArray
+ * (src/graph/graph_io.h:193-274) and PreprocessGraph::Preprocess
+ * (src/preprocess/preprocess_graph.h:12-92) on the device. d_pairs holds
+ * n_pairs (u, v) pairs of u32 ids < vertex_count; symmetric != 0 adds
+ * (v, u) for each. Self loops and duplicate entries are dropped and every
+ * list is sorted. drop_isolated != 0 (symmetric only) removes vertices with
+ * empty lists and renumbers the rest in order. out->row_ptrs
+ * (u64[out->vertices + 1]) and out->cols (u32[out->entries]) are
+ * pool-allocated; free them with vsgm_free. Needs two u64 keys per
+ * emitted entry plus the sort's workspace while it runs. */
+int vsgm_csr_from_pairs(vsgm_ctx* ctx, const uint32_t* d_pairs, uint64_t n_pairs, uint64_t vertex_count, int symmetric, int drop_isolated,
+                        vsgm_view_out* out);
+
 /* A candidate CSR produced by an op: offsets u64[path_num+1] and values
  * u32[count], both pool-allocated (free with vsgm_free). */
 typedef struct vsgm_csr_out {

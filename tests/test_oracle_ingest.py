@@ -1,0 +1,126 @@
+"""CPU: pinning the ingestion checker (oracle/ingest_ref.py) against a
+set-of-edges brute force and closed forms, then the host SNAP reader,
+isolated-vertex removal and the preprocess_graph tool against it."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import ingest_ref as I
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def brute(pairs, n, drop_isolated):
+    adj = {u: set() for u in range(n)}
+    for a, b in pairs:
+        if a != b:
+            adj[int(a)].add(int(b))
+            adj[int(b)].add(int(a))
+    keep = [u for u in range(n) if adj[u]] if drop_isolated else list(range(n))
+    new = {u: i for i, u in enumerate(keep)}
+    rp, cols = [0], []
+    for u in keep:
+        cols += sorted(new[w] for w in adj[u])
+        rp.append(len(cols))
+    return np.array(rp, np.uint64), np.array(cols, np.uint32)
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("drop", [False, True])
+def test_oracle_matches_brute_force(seed, drop):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 40))
+    m = int(rng.integers(0, 120))
+    pairs = rng.integers(0, n, size=(m, 2))
+    rp, cols = I.csr_from_pairs(pairs, n, True, drop)
+    brp, bcols = brute(pairs, n, drop)
+    assert np.array_equal(rp, brp) and np.array_equal(cols, bcols)
+
+
+def test_oracle_closed_forms():
+    n = 7
+    # every ordered pair twice plus loops: the complete graph K_7
+    pairs = [(a, b) for a in range(n) for b in range(n)] * 2
+    rp, cols = I.csr_from_pairs(pairs, n)
+    assert rp[-1] == n * (n - 1) and np.all(np.diff(rp) == n - 1)
+    # a star on ids 10, 20, ..., 50 with centre 30 inside 60 ids: 5 vertices remain
+    star = [(30, x) for x in (10, 20, 40, 50)] + [(30, 30)]
+    rp, cols = I.csr_from_pairs(star, 60, True, True)
+    assert len(rp) == 6 and list(np.diff(rp)) == [1, 1, 4, 1, 1]
+    assert list(cols[rp[2]:rp[3]]) == [0, 1, 3, 4]
+    # only loops: no entries
+    rp, cols = I.csr_from_pairs([(3, 3), (1, 1)], 5)
+    assert rp[-1] == 0 and len(rp) == 6
+
+
+def test_oracle_parse_snap():
+    text = "# comment\n% other\n\n5\t7\n7 9 1.5\n12\t5\n"
+    pairs, V = I.parse_snap(text)
+    assert V == 8 and pairs.tolist() == [[0, 2], [2, 4], [7, 0]]
+
+
+def snap_text(rng, n, m, offset):
+    lines = ["# Directed graph (each unordered pair of nodes is saved once)", "# FromNodeId\tToNodeId"]
+    for a, b in rng.integers(0, n, size=(m, 2)):
+        lines.append(f"{a + offset}\t{b + offset}")
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_host_read_snap_matches_oracle(vsgm, tmp_path, threads):
+    rng = np.random.default_rng(threads)
+    text = snap_text(rng, 300, 900, 17) + "5\t5\n"  # a loop below the smallest id shifts every id
+    path = tmp_path / "g.txt"
+    path.write_text(text)
+    g = vsgm.Graph.read_snap(str(path), threads=threads)
+    pairs, V = I.parse_snap(text)
+    rp, cols = I.csr_from_pairs(pairs, V, True, True)
+    grp, gcols = g.csr()
+    assert np.array_equal(grp, rp) and np.array_equal(gcols, cols)
+    assert g.validate() == ""
+
+
+def test_host_read_snap_errors(vsgm, tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1 2\n3\n")
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.Graph.read_snap(str(bad))
+    big = tmp_path / "big.txt"
+    big.write_text("1 4294967296\n")
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.Graph.read_snap(str(big))
+    empty = tmp_path / "empty.txt"
+    empty.write_text("# nothing\n")
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.Graph.read_snap(str(empty))
+    ok = tmp_path / "ok.txt"
+    ok.write_text("1 2\n")
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.Graph.read_snap(str(ok), directed=True)
+
+
+def test_drop_isolated_matches_oracle(vsgm):
+    rng = np.random.default_rng(4)
+    pairs = rng.integers(0, 500, size=(300, 2)).astype(np.uint32)
+    g = vsgm.Graph.from_edges(500, pairs).drop_isolated()
+    rp, cols = I.csr_from_pairs(pairs, 500, True, True)
+    grp, gcols = g.csr()
+    assert np.array_equal(grp, rp) and np.array_equal(gcols, cols)
+
+
+def test_preprocess_graph_cli(vsgm, tmp_path):
+    rng = np.random.default_rng(9)
+    text = snap_text(rng, 200, 700, 3)
+    path = tmp_path / "g.ungraph.txt"
+    path.write_text(text)
+    exe = os.path.join(ROOT, "vsgm_amd", "bin", "preprocess_graph")
+    out = subprocess.run([exe, "-f", str(path), "-d", "0", "-o", "1"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    g = vsgm.Graph.read_bin(str(path) + ".bin")
+    pairs, V = I.parse_snap(text)
+    rp, cols = I.csr_from_pairs(pairs, V, True, True)
+    grp, gcols = g.csr()
+    assert np.array_equal(grp, rp) and np.array_equal(gcols, cols)
+    assert subprocess.run([exe, "-f", str(path), "-o", "0"], capture_output=True, timeout=60).returncode != 0

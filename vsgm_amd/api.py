@@ -84,6 +84,8 @@ def _declare(L):
         "vsgm_graph_write_bin": ([P, C.c_char_p], C.c_int),
         "vsgm_graph_from_csr": ([u64, P, P, PP], C.c_int),
         "vsgm_graph_from_edges": ([u64, P, u64, PP], C.c_int),
+        "vsgm_graph_read_snap": ([C.c_char_p, C.c_int, C.c_int, C.c_int, PP], C.c_int),
+        "vsgm_graph_drop_isolated": ([P, PP], C.c_int),
         "vsgm_graph_rmat": ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, u64, C.c_int, PP], C.c_int),
         "vsgm_graph_chung_lu": ([u64, u64, C.c_double, u64, C.c_int, PP], C.c_int),
         "vsgm_graph_info": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], C.c_int),
@@ -141,6 +143,7 @@ def _declare(L):
         "vsgm_last_error": ([P], C.c_char_p),
         "vsgm_executor_free": ([P], None),
         "vsgm_pm_pipeline": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
+        "vsgm_preprocess_graph": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -209,6 +212,18 @@ class Graph(_Handle):
         h = C.c_void_p()
         _check(lib().vsgm_graph_from_edges(vertex_count, _ptr(pr), pr.shape[0], C.byref(h)), "from_edges")
         return cls(h.value)
+
+    @classmethod
+    def read_snap(cls, path: str, directed: bool = False, device: int = -1, threads: int = 0) -> "Graph":
+        """SNAP edge list -> cleaned graph without isolated vertices; device >= 0 builds the CSR on that GPU."""
+        h = C.c_void_p()
+        _check(lib().vsgm_graph_read_snap(path.encode(), int(directed), device, threads, C.byref(h)), f"read {path}")
+        return cls(h.value)
+
+    def drop_isolated(self) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().vsgm_graph_drop_isolated(self._h, C.byref(h)), "drop_isolated")
+        return Graph(h.value)
 
     @classmethod
     def rmat(cls, scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1, threads: int = 0) -> "Graph":
@@ -594,3 +609,10 @@ def pm_pipeline(argv: Sequence[str]) -> int:
     args = ["pm_pipeline"] + list(argv)
     arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
     return lib().vsgm_pm_pipeline(len(args), arr)
+
+
+def preprocess_graph(argv: Sequence[str]) -> int:
+    """Run the preprocess_graph tool in-process with reference-style flags."""
+    args = ["preprocess_graph"] + list(argv)
+    arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+    return lib().vsgm_preprocess_graph(len(args), arr)

@@ -7,6 +7,7 @@
 
 #include "executor.h"
 #include "graph.h"
+#include "ingest.h"
 #include "plan.h"
 #include "query.h"
 #include "view.h"
@@ -80,6 +81,14 @@ int vsgm_graph_from_edges(uint64_t vertex_count, const uint32_t* pairs, uint64_t
     }
     *out = new vsgm_graph{vsgm::BuildFromEdges(vertex_count, edges)};
   });
+}
+
+int vsgm_graph_read_snap(const char* path, int directed, int device, int threads, vsgm_graph** out) {
+  return Guard([&] { *out = new vsgm_graph{vsgm::ReadSNAPFile(path, directed != 0, device, threads)}; });
+}
+
+int vsgm_graph_drop_isolated(const vsgm_graph* g, vsgm_graph** out) {
+  return Guard([&] { *out = new vsgm_graph{vsgm::DropIsolated(g->g)}; });
 }
 
 int vsgm_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads, vsgm_graph** out) {

@@ -68,6 +68,10 @@ class OpStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("work", C.c_uint64), ("outputs", C.c_uint64), ("adj_entries", C.c_uint64)]
 
 
+class ViewOut(C.Structure):
+    _fields_ = [("row_ptrs", C.c_void_p), ("cols", C.c_void_p), ("entries", C.c_uint64), ("vertices", C.c_uint64)]
+
+
 class HubIndex(C.Structure):
     _fields_ = [("slot", C.c_void_p), ("bits", C.c_void_p), ("words", C.c_uint64), ("n_hubs", C.c_uint32)]
 
@@ -101,6 +105,7 @@ def _declare(L):
         "vsgm_last_op_stats": ([P, C.POINTER(OpStats)], i),
         "vsgm_hub_index_build": ([P, P, P, u64, P, C.c_uint32, C.POINTER(HubIndex)], i),
         "vsgm_hub_index_free": ([P, C.POINTER(HubIndex)], i),
+        "vsgm_csr_from_pairs": ([P, P, u64, u64, i, i, C.POINTER(ViewOut)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -226,3 +231,10 @@ class DeviceContext:
 
     def hub_index_free(self, index: HubIndex):
         self.check(hiplib().vsgm_hub_index_free(self.h, C.byref(index)), "hub index free")
+
+    def csr_from_pairs(self, pairs: DeviceArray, vertex_count: int, symmetric: bool = True, drop_isolated: bool = False):
+        """Device CSR of an edge list (pairs: u32 device array of 2n ids); returns (row_ptrs, cols) device arrays."""
+        out = ViewOut()
+        self.check(hiplib().vsgm_csr_from_pairs(self.h, pairs.ptr, pairs.n // 2, vertex_count, int(symmetric), int(drop_isolated),
+                                                C.byref(out)), "csr_from_pairs")
+        return self.adopt(out.row_ptrs, out.vertices + 1, np.uint64), self.adopt(out.cols, out.entries, np.uint32)
