@@ -170,6 +170,25 @@ void vsgm_executor_free(vsgm_executor* e);
 /* ---- the pm_pipeline driver (src/app/pattern_matching_pipeline.cu:27-201) */
 int vsgm_pm_pipeline(int argc, char** argv);
 
+/* ---- view-bin packing (src/tools/view_packing.cc:141-324, src/tools/view_bin.h:65-171):
+ * packs the sources of each first-layer cluster of `partition` (ids in
+ * [0, first_layer), or [0, first_layer * second_layer) for a two-layer
+ * partition) into bins of at most max_bin_entries list entries (their
+ * hop-1 balls, as the reference's packing counts them), at most pool_size
+ * bins open per cluster, sources in vertex order or by decreasing degree
+ * (degree_order), then merges the bins next-fit in increasing size. Writes
+ * the merged bin of every vertex to out_assign (int32[V]) and the bin count
+ * to *out_bins; out_sources / out_sizes (u64[cap], may be NULL) receive
+ * each bin's source count and size; a cap below the bin count is an error
+ * after *out_bins is set. Fails if one source's view alone exceeds the limit. */
+int vsgm_view_pack(const vsgm_graph* g, int hop, const int32_t* partition, int first_layer, int second_layer, uint64_t max_bin_entries,
+                   int pool_size, int degree_order, int threads, int32_t* out_assign, int* out_bins, uint64_t* out_sources,
+                   uint64_t* out_sizes, int cap);
+/* The reference's view_packing tool: -gf graph.bin -pf partition.<K>[x<M>]
+ * -h hop -m GB -t K -s pool -o degree_order -d dump; writes
+ * <graph>.<hop>.hop.vbstat and .vbmap. */
+int vsgm_view_packing(int argc, char** argv);
+
 /* ---- the preprocess_graph tool (src/preprocess/preprocess_graph.cc:9-36):
  * -f SNAP file, -d directed (0 only), -o 1 (.bin), plus -g GPU (>= 0 builds
  * the CSR there) and -t threads; writes <file>.bin */

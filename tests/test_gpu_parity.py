@@ -219,14 +219,15 @@ def test_pm_pipeline_cli(vsgm, oracle, gpu, tmp_path):
 
 
 def test_pm_pipeline_cli_device_views(vsgm, oracle, gpu, tmp_path):
-    """-gk partitions with the GPU k-means instead of the .vbmap file; -dv 1
-    builds every view on the consumer's GPU. Counts stay those of the graph."""
+    """-gk partitions with the GPU k-means instead of the .vbmap file, -gp
+    packs its clusters into size-bounded bins, -dv 1 builds every view on the
+    consumer's GPU. Counts stay those of the graph."""
     g = vsgm.Graph.rmat(10, 16, seed=13)
     path = str(tmp_path / "rmat10.bin")
     g.write_bin(path)
     exe = os.path.join(ROOT, "vsgm_amd", "bin", "pm_pipeline")
     want = oracle_counts(oracle, g, [vsgm.Plan.optimize(vsgm.Query.clique(4))])[0]
-    for extra in (["-gk", "3"], ["-gk", "3", "-dv", "1"], ["-gk", "1", "-dv", "1"]):
+    for extra in (["-gk", "3"], ["-gk", "3", "-dv", "1"], ["-gk", "1", "-dv", "1"], ["-gk", "3", "-gp", "2", "-dv", "1", "-m", "0.0001"]):
         print("[gpu] pm_pipeline", extra, flush=True)
         out = subprocess.run([exe, "-f", path, "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4", "-kc", "4"] + extra,
                              capture_output=True, text=True, timeout=120)

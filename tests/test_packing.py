@@ -1,0 +1,100 @@
+"""CPU: view-bin packing (vsgm_amd/csrc/host/packing.cc) against the
+reference's own view_packing tool — byte-identical .vbmap and .vbstat on
+the committed vectors (tests/golden/packing_vectors.json, written by
+tests/golden/make_packing_vectors.py from oracle/_ref/view_packing), and
+live on further seeded cases where that build is present — plus the
+library call, error paths and invariants."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TOOL = os.path.join(ROOT, "vsgm_amd", "bin", "view_packing")
+REF_TOOL = os.path.join(ROOT, "oracle", "_ref", "view_packing")
+VECTORS = json.load(open(os.path.join(ROOT, "tests", "golden", "packing_vectors.json")))
+
+
+def setup_case(vsgm, d, scale, ef, gseed, part, first, second):
+    g = vsgm.Graph.rmat(scale, ef, seed=gseed)
+    gf = os.path.join(d, "g.bin")
+    g.write_bin(gf)
+    pf = f"{gf}.kmeans.{first}x{second}" if second else f"{gf}.kmeans.{first}"
+    np.savetxt(pf, np.asarray(part), fmt="%d")
+    return g, gf, pf
+
+
+def run_tool(tool, gf, pf, first, hop, mem, pool, order):
+    return subprocess.run([tool, "-gf", gf, "-pf", pf, "-h", str(hop), "-m", repr(float(mem)), "-t", str(first), "-s", str(pool), "-o",
+                           str(order), "-d", "1"], capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("i", range(len(VECTORS)))
+def test_golden_vectors(vsgm, tmp_path, i):
+    v = VECTORS[i]
+    scale, ef, gseed, _pseed, first, second, hop, mem, pool, order = v["case"]
+    g, gf, pf = setup_case(vsgm, str(tmp_path), scale, ef, gseed, v["partition"], first, second)
+    out = run_tool(TOOL, gf, pf, first, hop, mem, pool, order)
+    assert out.returncode == 0, out.stderr
+    base = f"{gf}.{hop}.hop"
+    got = [int(x) for x in open(base + ".vbmap").read().split()]
+    assert got[0] == v["bins"] and got[1:] == v["assignment"]
+    assert open(base + ".vbstat").read() == v["vbstat"]
+    # the library call gives the same bins
+    max_entries = int((mem * 1000 * 1000 * 1000) / 4)
+    bins, assign, sources, sizes = vsgm.pack_view_bins(g, hop, v["partition"], first, second, max_entries, pool, bool(order))
+    assert bins == v["bins"] and assign.tolist() == v["assignment"]
+    assert int(sources.sum()) == g.vertex_count
+    assert np.array_equal(np.bincount(assign, minlength=bins), sources)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TOOL), reason="oracle/_ref/view_packing is built only where the reference tree exists")
+@pytest.mark.parametrize("seed", range(6))
+def test_live_against_reference_tool(vsgm, tmp_path, seed):
+    rng = np.random.default_rng(100 + seed)
+    scale, ef = 11, int(rng.choice([4, 8]))
+    first = int(rng.integers(1, 6))
+    second = int(rng.choice([0, 0, 2, 3]))
+    hop = int(rng.choice([1, 2, 2, 3]))
+    pool = int(rng.integers(1, 5))
+    order = int(rng.integers(0, 2))
+    g = vsgm.Graph.rmat(scale, ef, seed=seed + 7)
+    rp, cl = g.csr()
+    part = rng.integers(0, first * (second or 1), g.vertex_count)
+    deg = np.diff(rp).astype(np.int64)
+    ball2 = int((deg + np.add.reduceat(np.append(deg[cl], 0), rp[:-1].astype(np.int64)) * (deg > 0)).max())
+    mem = float({1: 4 * 1.5 * deg.max(), 2: 4 * 1.3 * ball2, 3: 4 * 1.5 * len(cl)}[hop] / 1e9)
+    outs = []
+    for name, tool in (("ref", REF_TOOL), ("me", TOOL)):
+        d = tmp_path / name
+        d.mkdir()
+        _, gf, pf = setup_case(vsgm, str(d), scale, ef, seed + 7, part, first, second)
+        r = run_tool(tool, gf, pf, first, hop, mem, pool, order)
+        base = f"{gf}.{hop}.hop"
+        outs.append((r.returncode == 0, open(base + ".vbmap").read() if r.returncode == 0 else None,
+                     open(base + ".vbstat").read() if r.returncode == 0 else None))
+    bins = outs[0][1].split()[0] if outs[0][0] else "-"
+    print(f"case scale={scale} ef={ef} layers={first}x{second} hop={hop} pool={pool} order={order}: bins={bins}")
+    assert outs[0] == outs[1]
+
+
+def test_errors(vsgm, tmp_path):
+    g, gf, pf = setup_case(vsgm, str(tmp_path), 9, 8, 1, np.zeros(512, int), 1, 0)
+    # a hub's 2-hop view cannot fit a 100-entry bin
+    assert run_tool(TOOL, gf, pf, 1, 2, 4e-7, 1, 0).returncode == 1
+    # -t must match the partition's first layer
+    assert run_tool(TOOL, gf, pf, 2, 2, 1.0, 1, 0).returncode == 2
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.pack_view_bins(g, 2, np.full(g.vertex_count, 3), 2)
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.pack_view_bins(g, 2, np.zeros(g.vertex_count), 1, 0, 100)
+
+
+def test_one_bin_when_everything_fits(vsgm):
+    g = vsgm.Graph.rmat(9, 8, seed=2)
+    part = np.random.default_rng(0).integers(0, 3, g.vertex_count)
+    bins, assign, sources, sizes = vsgm.pack_view_bins(g, 2, part, 3, 0, 1 << 40)
+    # one bin per cluster, merged into one
+    assert bins == 1 and not assign.any() and sources.tolist() == [g.vertex_count]

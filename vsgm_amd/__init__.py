@@ -18,6 +18,9 @@ from .api import (  # noqa: F401
     VsgmError,
     device_count,
     pm_pipeline,
+    preprocess_graph,
+    view_packing,
+    pack_view_bins,
     library_paths,
     OPS,
 )

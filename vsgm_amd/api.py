@@ -144,6 +144,9 @@ def _declare(L):
         "vsgm_executor_free": ([P], None),
         "vsgm_pm_pipeline": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
         "vsgm_preprocess_graph": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
+        "vsgm_view_packing": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
+        "vsgm_view_pack": ([P, C.c_int, P, C.c_int, C.c_int, u64, C.c_int, C.c_int, C.c_int, P, C.POINTER(C.c_int), P, P, C.c_int],
+                           C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -616,3 +619,27 @@ def preprocess_graph(argv: Sequence[str]) -> int:
     args = ["preprocess_graph"] + list(argv)
     arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
     return lib().vsgm_preprocess_graph(len(args), arr)
+
+
+def view_packing(argv: Sequence[str]) -> int:
+    """Run the view_packing tool in-process with reference-style flags."""
+    args = ["view_packing"] + list(argv)
+    arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+    return lib().vsgm_view_packing(len(args), arr)
+
+
+def pack_view_bins(graph: "Graph", hop: int, partition, first_layer: int, second_layer: int = 0, max_bin_entries: int = 2500000000,
+                   pool_size: int = 1, degree_order: bool = False, threads: int = 0):
+    """Pack a vertex partition into view bins; returns (bins, assignment int32[V], sources u64[bins], sizes u64[bins])."""
+    part = np.ascontiguousarray(partition, dtype=np.int32)
+    V = graph.vertex_count
+    if part.shape != (V,):
+        raise VsgmError("partition must hold one id per vertex")
+    assign = np.zeros(V, np.int32)
+    bins = C.c_int()
+    cap = V + 1
+    src, size = np.zeros(cap, np.uint64), np.zeros(cap, np.uint64)
+    _check(lib().vsgm_view_pack(graph._h, hop, _ptr(part), first_layer, second_layer, max_bin_entries, pool_size, int(degree_order),
+                                threads, _ptr(assign), C.byref(bins), _ptr(src), _ptr(size), cap), "view_pack")
+    n = bins.value
+    return n, assign, src[:n].copy(), size[:n].copy()

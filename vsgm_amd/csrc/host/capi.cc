@@ -8,6 +8,7 @@
 #include "executor.h"
 #include "graph.h"
 #include "ingest.h"
+#include "packing.h"
 #include "plan.h"
 #include "query.h"
 #include "view.h"
@@ -226,6 +227,22 @@ int vsgm_plan_serialize(const vsgm_plan* p, char* buf, size_t len, size_t* neede
 }
 
 void vsgm_plan_free(vsgm_plan* p) { delete p; }
+
+int vsgm_view_pack(const vsgm_graph* g, int hop, const int32_t* partition, int first_layer, int second_layer, uint64_t max_bin_entries,
+                   int pool_size, int degree_order, int threads, int32_t* out_assign, int* out_bins, uint64_t* out_sources,
+                   uint64_t* out_sizes, int cap) {
+  return Guard([&] {
+    VSGM_CHECK(g && partition && out_assign && out_bins, "null argument");
+    const uint64_t V = g->g.vertex_count;
+    vsgm::PackedBins p = vsgm::PackViewBins(g->g, hop, std::vector<int>(partition, partition + V), first_layer, second_layer,
+                                            max_bin_entries, pool_size, degree_order != 0, threads);
+    *out_bins = p.bins;
+    VSGM_CHECK((!out_sources && !out_sizes) || cap >= p.bins, "bin statistics need room for " + std::to_string(p.bins) + " bins");
+    std::copy(p.assignment.begin(), p.assignment.end(), out_assign);
+    if (out_sources) std::copy(p.sources.begin(), p.sources.end(), out_sources);
+    if (out_sizes) std::copy(p.sizes.begin(), p.sizes.end(), out_sizes);
+  });
+}
 
 int vsgm_vbmap_write(const char* path, int n_bins, const int* assignment, uint64_t vertex_count) {
   return Guard([&] { vsgm::WriteViewBinPartition(path, n_bins, std::vector<int>(assignment, assignment + vertex_count)); });

@@ -19,6 +19,7 @@
 
 #include "executor.h"
 #include "graph.h"
+#include "packing.h"
 #include "plan.h"
 #include "query.h"
 #include "view.h"
@@ -128,6 +129,9 @@ int Run(int argc, char** argv) {
   const int gpu_k_iters = fl.i("-gki", 10);
   const int gpu_k_seed = fl.i("-gks", 1);
   const int device_views = fl.i("-dv", 0);
+  // -gp P packs the k-means clusters into bins of at most -m GB of list
+  // entries with P bins open per cluster (the view_packing tool's step).
+  const int gpu_pack = fl.i("-gp", 0);
 
   int gpus = 0;
   vsgm_device_count(&gpus);
@@ -166,8 +170,16 @@ int Run(int argc, char** argv) {
   if (gpu_k > 0) {
     t0 = std::chrono::steady_clock::now();
     std::vector<int> assignment = GpuKMeans(graph, gpu_k, gpu_k_iters, (uint64_t)gpu_k_seed);
-    vbm.SetPartition(gpu_k, assignment);
     std::cout << "GPU k-means: " << gpu_k << " clusters in " << Ms(t0) << " ms" << std::endl;
+    if (gpu_pack > 0) {
+      t0 = std::chrono::steady_clock::now();
+      const uint64_t max_entries = (uint64_t)((mem * 1000 * 1000 * 1000) / sizeof(uintV));
+      PackedBins packed = PackViewBins(graph, hop, assignment, gpu_k, 0, max_entries, gpu_pack, false, thread_num);
+      vbm.SetPartition(packed.bins, packed.assignment);
+      std::cout << "packed into " << packed.bins << " view bins in " << Ms(t0) << " ms" << std::endl;
+    } else {
+      vbm.SetPartition(gpu_k, assignment);
+    }
   } else {
     vbm.LoadViewBinPartition(ViewBinPartitionName(filename, hop));
   }
