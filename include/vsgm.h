@@ -189,6 +189,20 @@ int vsgm_view_pack(const vsgm_graph* g, int hop, const int32_t* partition, int f
  * <graph>.<hop>.hop.vbstat and .vbmap. */
 int vsgm_view_packing(int argc, char** argv);
 
+/* ---- the kmeans tool (src/tools/kmeans.cc:636-694) on the GPU: -f graph.bin
+ * -k K -i iterations -o output prefix (default <graph>.kmeans.) -d dump
+ * -rc reorder -init 0, plus -g GPU and -s seed (the reference seeds with
+ * the clock); writes one cluster id per line to <prefix><K>, empty clusters
+ * removed. Multi-level "axb" / "a^x" clustering is rejected. */
+int vsgm_kmeans_tool(int argc, char** argv);
+/* The tool's two host steps on an assignment (int32[V], ids < k):
+ * renumber clusters by decreasing count of distinct vertices in their
+ * members' closed neighbourhoods (ReorderCentroids, kmeans.cc:435-452), and
+ * drop empty clusters keeping the order (RemoveEmptyClusters,
+ * kmeans.cc:491-510; *kept = clusters left). */
+int vsgm_clusters_reorder(const vsgm_graph* g, int k, int32_t* assignment);
+int vsgm_clusters_compact(int k, int32_t* assignment, uint64_t vertex_count, int* kept);
+
 /* ---- the preprocess_graph tool (src/preprocess/preprocess_graph.cc:9-36):
  * -f SNAP file, -d directed (0 only), -o 1 (.bin), plus -g GPU (>= 0 builds
  * the CSR there) and -t threads; writes <file>.bin */

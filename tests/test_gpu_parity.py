@@ -234,3 +234,35 @@ def test_pm_pipeline_cli_device_views(vsgm, oracle, gpu, tmp_path):
         assert out.returncode == 0, out.stderr[-2000:]
         line = [x for x in out.stdout.splitlines() if x.startswith("total count for query 0:")][0]
         assert int(line.split(":")[1]) == want
+
+
+def test_offline_tool_chain(vsgm, oracle, gpu, tmp_path):
+    """SNAP text -> preprocess_graph -g 0 -> kmeans (GPU) -> view_packing ->
+    pm_pipeline, the reference's tool chain, ends in the graph's count."""
+    g0 = vsgm.Graph.rmat(10, 16, seed=21)
+    rp, cl = g0.csr()
+    src = np.repeat(np.arange(g0.vertex_count), np.diff(rp).astype(np.int64))
+    keep = src < cl
+    text = "# FromNodeId\tToNodeId\n" + "".join(f"{a + 3}\t{b + 3}\n" for a, b in zip(src[keep], cl[keep]))
+    path = str(tmp_path / "g.ungraph.txt")
+    open(path, "w").write(text)
+    bindir = os.path.join(ROOT, "vsgm_amd", "bin")
+
+    def run(tool, *args):
+        print("[gpu]", tool, *args, flush=True)
+        out = subprocess.run([os.path.join(bindir, tool)] + list(args), capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr[-2000:]
+        return out.stdout
+
+    run("preprocess_graph", "-f", path, "-d", "0", "-g", "0")
+    g = vsgm.Graph.read_bin(path + ".bin")
+    grp, gcl = g.csr()
+    deg = np.diff(grp).astype(np.int64)
+    ball2 = int((deg + np.add.reduceat(np.append(deg[gcl], 0), grp[:-1].astype(np.int64)) * (deg > 0)).max())
+    run("kmeans", "-f", path + ".bin", "-k", "3", "-i", "5", "-rc", "1")
+    assert len(open(path + ".bin.kmeans.3").read().split()) == g.vertex_count
+    run("view_packing", "-gf", path + ".bin", "-pf", path + ".bin.kmeans.3", "-h", "2", "-m", repr(4 * 1.3 * ball2 / 1e9), "-t", "3",
+        "-s", "2")
+    out = run("pm_pipeline", "-f", path + ".bin", "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4", "-kc", "4")
+    line = [x for x in out.splitlines() if x.startswith("total count for query 0:")][0]
+    assert int(line.split(":")[1]) == oracle_counts(oracle, g, [vsgm.Plan.optimize(vsgm.Query.clique(4))])[0]

@@ -98,3 +98,17 @@ def test_one_bin_when_everything_fits(vsgm):
     bins, assign, sources, sizes = vsgm.pack_view_bins(g, 2, part, 3, 0, 1 << 40)
     # one bin per cluster, merged into one
     assert bins == 1 and not assign.any() and sources.tolist() == [g.vertex_count]
+
+
+def test_cluster_reorder_and_compaction(vsgm):
+    # two disjoint triangles {0,1,2}, {3,4,5} and a 5-vertex path 6..10
+    edges = [(0, 1), (1, 2), (0, 2), (3, 4), (4, 5), (3, 5), (6, 7), (7, 8), (8, 9), (9, 10)]
+    g = vsgm.Graph.from_edges(11, edges)
+    a = [0, 0, 0, 2, 2, 2, 3, 3, 3, 3, 3]  # cluster 1 empty
+    # distinct closed-neighbourhood vertices: cluster 0 -> 3, 2 -> 3, 3 -> 5, 1 -> 0
+    r = vsgm.reorder_clusters(g, 4, a)
+    assert r.tolist() == [1, 1, 1, 2, 2, 2, 0, 0, 0, 0, 0]
+    kept, c = vsgm.compact_clusters(4, a)
+    assert kept == 3 and c.tolist() == [0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 2]
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.compact_clusters(2, a)

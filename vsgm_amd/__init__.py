@@ -21,6 +21,9 @@ from .api import (  # noqa: F401
     preprocess_graph,
     view_packing,
     pack_view_bins,
+    kmeans_tool,
+    reorder_clusters,
+    compact_clusters,
     library_paths,
     OPS,
 )

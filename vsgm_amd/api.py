@@ -145,6 +145,9 @@ def _declare(L):
         "vsgm_pm_pipeline": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
         "vsgm_preprocess_graph": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
         "vsgm_view_packing": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
+        "vsgm_kmeans_tool": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
+        "vsgm_clusters_reorder": ([P, C.c_int, P], C.c_int),
+        "vsgm_clusters_compact": ([C.c_int, P, u64, C.POINTER(C.c_int)], C.c_int),
         "vsgm_view_pack": ([P, C.c_int, P, C.c_int, C.c_int, u64, C.c_int, C.c_int, C.c_int, P, C.POINTER(C.c_int), P, P, C.c_int],
                            C.c_int),
     }
@@ -643,3 +646,23 @@ def pack_view_bins(graph: "Graph", hop: int, partition, first_layer: int, second
                                 threads, _ptr(assign), C.byref(bins), _ptr(src), _ptr(size), cap), "view_pack")
     n = bins.value
     return n, assign, src[:n].copy(), size[:n].copy()
+
+
+def kmeans_tool(argv: Sequence[str]) -> int:
+    """Run the kmeans tool (GPU) in-process with reference-style flags."""
+    args = ["kmeans"] + list(argv)
+    arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+    return lib().vsgm_kmeans_tool(len(args), arr)
+
+
+def reorder_clusters(graph: "Graph", k: int, assignment) -> np.ndarray:
+    a = np.array(assignment, dtype=np.int32)
+    _check(lib().vsgm_clusters_reorder(graph._h, k, _ptr(a)), "clusters_reorder")
+    return a
+
+
+def compact_clusters(k: int, assignment):
+    a = np.array(assignment, dtype=np.int32)
+    kept = C.c_int()
+    _check(lib().vsgm_clusters_compact(k, _ptr(a), a.size, C.byref(kept)), "clusters_compact")
+    return kept.value, a

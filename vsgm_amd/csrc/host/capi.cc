@@ -8,6 +8,7 @@
 #include "executor.h"
 #include "graph.h"
 #include "ingest.h"
+#include "kmeans_tool.h"
 #include "packing.h"
 #include "plan.h"
 #include "query.h"
@@ -241,6 +242,27 @@ int vsgm_view_pack(const vsgm_graph* g, int hop, const int32_t* partition, int f
     std::copy(p.assignment.begin(), p.assignment.end(), out_assign);
     if (out_sources) std::copy(p.sources.begin(), p.sources.end(), out_sources);
     if (out_sizes) std::copy(p.sizes.begin(), p.sizes.end(), out_sizes);
+  });
+}
+
+int vsgm_clusters_reorder(const vsgm_graph* g, int k, int32_t* assignment) {
+  return Guard([&] {
+    VSGM_CHECK(g && assignment && k >= 1, "bad arguments");
+    const uint64_t V = g->g.vertex_count;
+    std::vector<int> a(assignment, assignment + V);
+    for (int x : a) VSGM_CHECK(x >= 0 && x < k, "cluster id out of range");
+    vsgm::ReorderClusters(g->g, k, a);
+    std::copy(a.begin(), a.end(), assignment);
+  });
+}
+
+int vsgm_clusters_compact(int k, int32_t* assignment, uint64_t vertex_count, int* kept) {
+  return Guard([&] {
+    VSGM_CHECK(assignment && kept && k >= 1, "bad arguments");
+    std::vector<int> a(assignment, assignment + vertex_count);
+    for (int x : a) VSGM_CHECK(x >= 0 && x < k, "cluster id out of range");
+    *kept = vsgm::RemoveEmptyClusters(k, a);
+    std::copy(a.begin(), a.end(), assignment);
   });
 }
 

@@ -19,12 +19,12 @@
 
 #include "executor.h"
 #include "graph.h"
+#include "kmeans_tool.h"
 #include "packing.h"
 #include "plan.h"
 #include "query.h"
 #include "view.h"
 #include "vsgm.h"
-#include "vsgm_partition.h"
 
 namespace vsgm {
 namespace {
@@ -79,30 +79,6 @@ struct Flags {
 
 double Ms(std::chrono::steady_clock::time_point a) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
-}
-
-// Vertex partition by the GPU k-means (include/vsgm_partition.h) on device 0.
-std::vector<int> GpuKMeans(const Graph& g, int k, int iters, uint64_t seed) {
-  vsgm_ctx* ctx = nullptr;
-  if (vsgm_ctx_create(0, 0, &ctx) != VSGM_OK) throw Error("GPU k-means needs a GPU");
-  void *rp = nullptr, *cols = nullptr;
-  std::vector<int32_t> assign(g.vertex_count);
-  std::vector<uint64_t> sizes(k);
-  auto check = [&](int s, const char* what) {
-    if (s != VSGM_OK) {
-      std::string msg = std::string(what) + " failed: " + vsgm_last_error(ctx);
-      vsgm_ctx_destroy(ctx);
-      throw Error(msg);
-    }
-  };
-  check(vsgm_alloc(ctx, (g.vertex_count + 1) * sizeof(uint64_t), &rp), "k-means allocation");
-  check(vsgm_alloc(ctx, std::max<size_t>(g.cols.size(), 1) * sizeof(uint32_t), &cols), "k-means allocation");
-  check(vsgm_h2d(ctx, rp, g.row_ptrs.data(), (g.vertex_count + 1) * sizeof(uint64_t)), "graph upload");
-  check(vsgm_h2d(ctx, cols, g.cols.data(), g.cols.size() * sizeof(uint32_t)), "graph upload");
-  check(vsgm_kmeans(ctx, (const uint64_t*)rp, (const uint32_t*)cols, g.vertex_count, k, iters, seed, assign.data(), sizes.data(), nullptr),
-        "k-means");
-  vsgm_ctx_destroy(ctx);  // frees the pool
-  return std::vector<int>(assign.begin(), assign.end());
 }
 
 int Run(int argc, char** argv) {
