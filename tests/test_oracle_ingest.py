@@ -124,3 +124,22 @@ def test_preprocess_graph_cli(vsgm, tmp_path):
     grp, gcols = g.csr()
     assert np.array_equal(grp, rp) and np.array_equal(gcols, cols)
     assert subprocess.run([exe, "-f", str(path), "-o", "0"], capture_output=True, timeout=60).returncode != 0
+
+
+def test_host_read_snap_tiny_file_many_threads(vsgm, tmp_path):
+    # fewer bytes than threads: every chunk cut but the last is empty
+    path = tmp_path / "tiny.txt"
+    path.write_text("1 2\n")
+    for threads in (1, 8, 64):
+        rp, cols = vsgm.Graph.read_snap(str(path), threads=threads).csr()
+        assert rp.tolist() == [0, 1, 2] and cols.tolist() == [1, 0]
+    crlf = tmp_path / "crlf.txt"
+    crlf.write_bytes(b"# c\r\n3\t4\r\n4 5 extra\r\n")
+    rp, cols = vsgm.Graph.read_snap(str(crlf), threads=4).csr()
+    assert rp.tolist() == [0, 1, 3, 4] and cols.tolist() == [1, 0, 2, 1]
+
+
+def test_drop_isolated_rejects_asymmetric_csr(vsgm):
+    g = vsgm.Graph.from_csr(np.array([0, 1, 1, 1], np.uint64), np.array([2], np.uint32))
+    with pytest.raises(vsgm.VsgmError):
+        g.drop_isolated()

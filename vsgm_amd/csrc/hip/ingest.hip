@@ -109,7 +109,7 @@ int vsgm_csr_from_pairs(vsgm_ctx* ctx, const uint32_t* d_pairs, uint64_t n_pairs
     return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "bad edge-list arguments");
   if (drop_isolated && !symmetric)
     return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "isolated-vertex removal needs a symmetric graph");
-  if (n_pairs > (1ull << 62)) return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "edge list too long");
+  if (n_pairs > (1ull << 59)) return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "edge list too long");
   const uint64_t V = vertex_count;
   const uint64_t N = symmetric ? 2 * n_pairs : n_pairs;
   hipStream_t s = vsgm_internal::stream(ctx);

@@ -100,7 +100,7 @@ EdgeList ReadSNAPEdges(const std::string& filename, int threads) {
   for (int t = 0; t <= nt; ++t) {
     uint64_t c = f.size() * (uint64_t)t / (uint64_t)nt;
     if (t > 0 && t < nt)
-      while (c < f.size() && f.data()[c - 1] != '\n') ++c;
+      while (c > 0 && c < f.size() && f.data()[c - 1] != '\n') ++c;
     cut[t] = c;
   }
   for (int t = 1; t <= nt; ++t) cut[t] = std::max(cut[t], cut[t - 1]);
@@ -138,6 +138,8 @@ EdgeList ReadSNAPEdges(const std::string& filename, int threads) {
 }
 
 Graph DropIsolated(const Graph& g) {
+  // every list entry must point at a vertex whose own list is not empty
+  for (uint32_t c : g.cols) VSGM_CHECK(g.row_ptrs[c + 1] > g.row_ptrs[c], "isolated-vertex removal needs a symmetric graph");
   std::vector<uint32_t> id(g.vertex_count);
   uint64_t kept = 0;
   for (uint64_t u = 0; u < g.vertex_count; ++u)
