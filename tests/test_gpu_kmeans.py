@@ -101,3 +101,19 @@ def test_kmeans_bins_make_smaller_views(vsgm, gpu):
     print(f"[gpu] R-MAT s14, k=8: view entries k-means {km} vs random {rnd} ({km / rnd:.3f}); objective {obj[0]:.1f} -> {obj[-1]:.1f}",
           flush=True)
     assert km < rnd
+
+
+def test_reference_self_test_pairs(vsgm, gpu):
+    """The reference's own k-means self-test (src/tools/kmeans.cc:697-771) on
+    the GPU: each pair (2i, 2i+1) of the 8 directed lists lands in one
+    cluster of 2, for every seed."""
+    from vsgm_amd.partition import kmeans
+
+    lists = [[2, 3], [2, 3], [4, 5], [4, 5], [6, 7], [6, 7], [0, 1], [0, 1]]
+    rp = np.zeros(9, np.uint64)
+    rp[1:] = np.cumsum([len(x) for x in lists])
+    g = vsgm.Graph.from_csr(rp, np.array([y for x in lists for y in x], np.uint32))
+    for seed in range(1, 9):
+        a, sizes, _ = kmeans(g, 4, iters=10, seed=seed)
+        assert all(a[i] == a[i + 1] for i in range(0, 8, 2)), (seed, a)
+        assert sorted(sizes.tolist()) == [2, 2, 2, 2]

@@ -62,3 +62,18 @@ def test_objective_does_not_decrease():
     rp, cols, _ = csr(n, [tuple(rng.integers(0, n, 2)) for _ in range(300)])
     _, _, obj = K.kmeans(rp, cols, 4, 6, first=5)
     assert all(b >= a - 1e-9 for a, b in zip(obj, obj[1:]))
+
+
+def test_reference_self_test_pairs():
+    """The reference's own k-means self-test (src/tools/kmeans.cc:697-771):
+    on the 8 directed lists {2,3},{2,3},{4,5},{4,5},{6,7},{6,7},{0,1},{0,1},
+    k = 4 with 10 rounds puts each pair (2i, 2i+1) in one cluster and every
+    cluster holds 2 vertices, whatever the first centroid."""
+    lists = [[2, 3], [2, 3], [4, 5], [4, 5], [6, 7], [6, 7], [0, 1], [0, 1]]
+    rp = np.zeros(9, np.uint64)
+    rp[1:] = np.cumsum([len(x) for x in lists])
+    cols = np.array([y for x in lists for y in x], np.uint32)
+    for first in range(8):
+        a, sizes, _ = K.kmeans(rp, cols, 4, 10, first)
+        assert all(a[i] == a[i + 1] for i in range(0, 8, 2)), (first, a)
+        assert sorted(sizes.tolist()) == [2, 2, 2, 2]
