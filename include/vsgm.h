@@ -49,6 +49,14 @@ int vsgm_graph_from_edges(uint64_t vertex_count, const uint32_t* pairs, uint64_t
  * device < 0 on the host; threads <= 0 uses all cores. directed != 0 is
  * rejected (the matcher reads undirected graphs). */
 int vsgm_graph_read_snap(const char* path, int directed, int device, int threads, vsgm_graph** out);
+/* Any graph file by suffix (GraphIO::ReadDataFile, src/graph/graph_io.h:42-55):
+ * .bin CSR, .txt SNAP (ids shifted by the smallest), .lvid large-id SNAP
+ * (every digit run an id, numbered by first appearance, graph_io.h:106-191).
+ * Text input is symmetrised and cleaned of self loops and parallel edges
+ * (the reference keeps them when it matches a .txt file directly);
+ * drop_isolated != 0 also removes isolated vertices. device >= 0 builds
+ * text input on that GPU. */
+int vsgm_graph_load(const char* path, int device, int threads, int drop_isolated, vsgm_graph** out);
 /* Copy of g without its isolated vertices, ids renumbered in order
  * (preprocess_graph.h:67-82). */
 int vsgm_graph_drop_isolated(const vsgm_graph* g, vsgm_graph** out);

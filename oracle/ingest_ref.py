@@ -5,7 +5,8 @@ TEST INFRASTRUCTURE ONLY: the product runs vsgm_amd/csrc/hip/ingest.hip and
 vsgm_amd/csrc/host/ingest.cc; this numpy version is the checker. It follows
 GraphIO::ReadSNAPFile (src/graph/graph_io.h:193-274: lines not starting with
 a digit skipped, ids shifted by the smallest id, both directions of each
-undirected edge), SortCSRArray, and PreprocessGraph::Preprocess
+undirected edge; ReadSNAPLargeVIDFile, :106-191, for large ids),
+SortCSRArray, and PreprocessGraph::Preprocess
 (src/preprocess/preprocess_graph.h:12-92: self loops and parallel edges
 dropped, then vertices with empty lists removed and the rest renumbered in
 order). Pinned by tests/test_oracle_ingest.py against a set-of-edges brute
@@ -27,6 +28,23 @@ def parse_snap(text: str):
     arr = np.array(rows, dtype=np.int64)
     lo, hi = int(arr.min()), int(arr.max())
     return (arr - lo).astype(np.uint32), hi - lo + 1
+
+
+def parse_snap_lvid(text: str):
+    """(pairs u32[n][2], vertex_count) of a large-id SNAP text
+    (graph_io.h:106-191): every digit run of a line that starts with a
+    digit is an id, numbered by first appearance; consecutive ids pair."""
+    import re
+
+    ids, ends = {}, []
+    for line in text.split("\n"):
+        if not line or not line[0].isdigit():
+            continue
+        for tok in re.findall(r"[0-9]+", line):
+            ends.append(ids.setdefault(tok, len(ids)))
+    if not ends or len(ends) % 2:
+        raise ValueError("no edges or an odd number of ids")
+    return np.array(ends, np.uint32).reshape(-1, 2), len(ids)
 
 
 def csr_from_pairs(pairs, vertex_count: int, symmetric: bool = True, drop_isolated: bool = False):

@@ -123,7 +123,7 @@ def test_preprocess_graph_cli(vsgm, tmp_path):
     rp, cols = I.csr_from_pairs(pairs, V, True, True)
     grp, gcols = g.csr()
     assert np.array_equal(grp, rp) and np.array_equal(gcols, cols)
-    assert subprocess.run([exe, "-f", str(path), "-o", "0"], capture_output=True, timeout=60).returncode != 0
+    assert subprocess.run([exe, "-f", str(path), "-o", "7"], capture_output=True, timeout=60).returncode != 0
 
 
 def test_host_read_snap_tiny_file_many_threads(vsgm, tmp_path):
@@ -143,3 +143,51 @@ def test_drop_isolated_rejects_asymmetric_csr(vsgm):
     g = vsgm.Graph.from_csr(np.array([0, 1, 1, 1], np.uint64), np.array([2], np.uint32))
     with pytest.raises(vsgm.VsgmError):
         g.drop_isolated()
+
+
+def test_oracle_parse_snap_lvid():
+    text = "# web graph\n100 2000000000000000000000\n2000000000000000000000\t7\n7,100\n007 7\n"
+    pairs, V = I.parse_snap_lvid(text)
+    # ids by first appearance: 100 -> 0, 2e21 -> 1, 7 -> 2, "007" -> 3 (a different string)
+    assert V == 4 and pairs.tolist() == [[0, 1], [1, 2], [2, 0], [3, 2]]
+    with pytest.raises(ValueError):
+        I.parse_snap_lvid("1 2 3\n")
+
+
+def test_load_by_suffix(vsgm, tmp_path):
+    rng = np.random.default_rng(11)
+    big = [f"{10**19 + int(a) * 7919}\t{10**19 + int(b) * 7919}" for a, b in rng.integers(0, 400, size=(1500, 2))]
+    lvid = tmp_path / "g.lvid"
+    lvid.write_text("# ids above 2^64\n" + "\n".join(big) + "\n")
+    pairs, V = I.parse_snap_lvid(lvid.read_text())
+    rp, cols = vsgm.Graph.load(str(lvid)).csr()
+    want = I.csr_from_pairs(pairs, V, True, False)
+    assert np.array_equal(rp, want[0]) and np.array_equal(cols, want[1])
+    # .txt keeps isolated vertices unless asked (ids must match a .vbmap made for the file)
+    txt = tmp_path / "g.txt"
+    txt.write_text("5 9\n9 12\n12 12\n")
+    rp, cols = vsgm.Graph.load(str(txt)).csr()
+    assert len(rp) == 9 and rp[-1] == 4 and cols.tolist() == [4, 0, 7, 4]
+    rp, cols = vsgm.Graph.load(str(txt), drop_isolated=True).csr()
+    assert rp.tolist() == [0, 1, 3, 4] and cols.tolist() == [1, 0, 2, 1]
+    g = vsgm.Graph.from_edges(50, rng.integers(0, 50, size=(20, 2)))
+    g.write_bin(str(tmp_path / "g.bin"))
+    a = vsgm.Graph.load(str(tmp_path / "g.bin"), drop_isolated=True).csr()
+    b = g.drop_isolated().csr()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.Graph.load(str(tmp_path / "g.unknown"))
+
+
+def test_preprocess_graph_text_outputs(vsgm, tmp_path):
+    path = tmp_path / "g.txt"
+    path.write_text("# c\n4 6\n6 8\n8 4\n8 8\n4 6\n10 12\n")
+    exe = os.path.join(ROOT, "vsgm_amd", "bin", "preprocess_graph")
+    for fmt in ("0", "2"):
+        out = subprocess.run([exe, "-f", str(path), "-o", fmt], capture_output=True, text=True, timeout=60)
+        assert out.returncode == 0, out.stderr
+    # ids 4,6,8,10,12 -> shifted 0..8 -> isolated removed: 0,2,4,6,8 -> 0..4
+    assert (tmp_path / "g.txt.snap.txt").read_text() == "0 1\n0 2\n1 2\n3 4\n"
+    assert (tmp_path / "g.txt.adj.txt").read_text() == "0\t1 2 \n1\t0 2 \n2\t0 1 \n3\t4 \n4\t3 \n"
+    assert (tmp_path / "g.txt.adj.txt.info").read_text() == "5 8"
+    assert subprocess.run([exe, "-f", str(path), "-o", "3"], capture_output=True, timeout=60).returncode == 2

@@ -86,6 +86,7 @@ def _declare(L):
         "vsgm_graph_from_edges": ([u64, P, u64, PP], C.c_int),
         "vsgm_graph_read_snap": ([C.c_char_p, C.c_int, C.c_int, C.c_int, PP], C.c_int),
         "vsgm_graph_drop_isolated": ([P, PP], C.c_int),
+        "vsgm_graph_load": ([C.c_char_p, C.c_int, C.c_int, C.c_int, PP], C.c_int),
         "vsgm_graph_rmat": ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, u64, C.c_int, PP], C.c_int),
         "vsgm_graph_chung_lu": ([u64, u64, C.c_double, u64, C.c_int, PP], C.c_int),
         "vsgm_graph_info": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], C.c_int),
@@ -224,6 +225,13 @@ class Graph(_Handle):
         """SNAP edge list -> cleaned graph without isolated vertices; device >= 0 builds the CSR on that GPU."""
         h = C.c_void_p()
         _check(lib().vsgm_graph_read_snap(path.encode(), int(directed), device, threads, C.byref(h)), f"read {path}")
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path: str, device: int = -1, threads: int = 0, drop_isolated: bool = False) -> "Graph":
+        """Graph file by suffix: .bin CSR, .txt SNAP, .lvid large-id SNAP."""
+        h = C.c_void_p()
+        _check(lib().vsgm_graph_load(path.encode(), device, threads, int(drop_isolated), C.byref(h)), f"load {path}")
         return cls(h.value)
 
     def drop_isolated(self) -> "Graph":

@@ -89,6 +89,10 @@ int vsgm_graph_read_snap(const char* path, int directed, int device, int threads
   return Guard([&] { *out = new vsgm_graph{vsgm::ReadSNAPFile(path, directed != 0, device, threads)}; });
 }
 
+int vsgm_graph_load(const char* path, int device, int threads, int drop_isolated, vsgm_graph** out) {
+  return Guard([&] { *out = new vsgm_graph{vsgm::LoadGraph(path, device, threads, drop_isolated != 0)}; });
+}
+
 int vsgm_graph_drop_isolated(const vsgm_graph* g, vsgm_graph** out) {
   return Guard([&] { *out = new vsgm_graph{vsgm::DropIsolated(g->g)}; });
 }

@@ -13,7 +13,9 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
+#include <unordered_map>
 
 #include <omp.h>
 
@@ -137,6 +139,42 @@ EdgeList ReadSNAPEdges(const std::string& filename, int threads) {
   return out;
 }
 
+EdgeList ReadSNAPLargeVIDEdges(const std::string& filename) {
+  MappedFile f(filename);
+  const char* t = f.data();
+  const uint64_t n = f.size();
+  std::unordered_map<std::string, uint32_t> ids;
+  std::vector<uint32_t> ends;
+  uint64_t i = 0;
+  while (i < n) {
+    uint64_t eol = i;
+    while (eol < n && t[eol] != '\n') ++eol;
+    if (eol > i && IsDigit(t[i])) {
+      // every maximal run of digits on the line is one id, numbered in order of first appearance
+      for (uint64_t p = i; p < eol;) {
+        if (!IsDigit(t[p])) {
+          ++p;
+          continue;
+        }
+        uint64_t q = p;
+        while (q < eol && IsDigit(t[q])) ++q;
+        auto it = ids.emplace(std::string(t + p, q - p), (uint32_t)ids.size()).first;
+        if (ids.size() > 0xFFFFFFFFull) throw Error(filename + ": more than 2^32-1 distinct vertex ids");
+        ends.push_back(it->second);
+        p = q;
+      }
+    }
+    i = eol + 1;
+  }
+  if (ends.size() % 2) throw Error(filename + ": odd number of vertex ids");
+  if (ends.empty()) throw Error(filename + ": no edge lines");
+  EdgeList out;
+  out.vertex_count = ids.size();
+  out.edges.resize(ends.size() / 2);
+  for (size_t e = 0; e < out.edges.size(); ++e) out.edges[e] = {ends[2 * e], ends[2 * e + 1]};
+  return out;
+}
+
 Graph DropIsolated(const Graph& g) {
   // every list entry must point at a vertex whose own list is not empty
   for (uint32_t c : g.cols) VSGM_CHECK(g.row_ptrs[c + 1] > g.row_ptrs[c], "isolated-vertex removal needs a symmetric graph");
@@ -191,6 +229,49 @@ Graph ReadSNAPFile(const std::string& filename, bool directed, int device, int t
   return DropIsolated(BuildFromEdges(el.vertex_count, el.edges, threads));
 }
 
+Graph LoadGraph(const std::string& filename, int device, int threads, bool drop_isolated) {
+  const std::string suffix = filename.substr(filename.rfind('.') + 1);
+  if (suffix == "bin") {
+    Graph g = ReadCSRBinFile(filename);
+    if (!drop_isolated) return g;  // moved, not copied
+    return DropIsolated(g);
+  }
+  EdgeList el;
+  if (suffix == "txt")
+    el = ReadSNAPEdges(filename, threads);
+  else if (suffix == "lvid")
+    el = ReadSNAPLargeVIDEdges(filename);
+  else
+    throw Error("unknown graph file suffix ." + suffix + " (expected .bin, .txt or .lvid)");
+  if (device >= 0) return BuildOnDevice(device, el.vertex_count, el.edges, drop_isolated);
+  Graph g = BuildFromEdges(el.vertex_count, el.edges, threads);
+  if (!drop_isolated) return g;
+  return DropIsolated(g);
+}
+
+void WriteSNAPText(const std::string& filename, const Graph& g) {
+  std::ofstream f(filename);
+  if (!f) throw Error("cannot write " + filename);
+  for (uint64_t u = 0; u < g.vertex_count; ++u)
+    for (uint64_t j = g.row_ptrs[u]; j < g.row_ptrs[u + 1]; ++j)
+      if (u < g.cols[j]) f << u << " " << g.cols[j] << "\n";
+  if (!f) throw Error("write to " + filename + " failed");
+}
+
+void WriteAdjText(const std::string& filename, const Graph& g) {
+  std::ofstream f(filename);
+  if (!f) throw Error("cannot write " + filename);
+  for (uint64_t u = 0; u < g.vertex_count; ++u) {
+    f << u << "\t";
+    for (uint64_t j = g.row_ptrs[u]; j < g.row_ptrs[u + 1]; ++j) f << g.cols[j] << " ";
+    f << "\n";
+  }
+  if (!f) throw Error("write to " + filename + " failed");
+  std::ofstream info(filename + ".info");
+  info << g.vertex_count << " " << g.cols.size();
+  if (!info) throw Error("write to " + filename + ".info failed");
+}
+
 namespace {
 
 struct Args {
@@ -214,7 +295,7 @@ double Ms(std::chrono::steady_clock::time_point a) {
 int PreprocessGraphMain(int argc, char** argv) {
   if (argc == 1) {
     std::cout << "./preprocess_graph -f GRAPH_FILENAME(SNAP format) -d DIRECTED -o OUTPUT_FORMAT [-g GPU] [-t THREADS]" << std::endl;
-    std::cout << "OUTPUT_FORMAT: 1 (Default) for .bin; -g >= 0 builds the CSR on that GPU" << std::endl;
+    std::cout << "OUTPUT_FORMAT: 0 for .snap.txt, 1 (Default) for .bin, 2 for .adj.txt; -g >= 0 builds the CSR on that GPU" << std::endl;
     return -1;
   }
   Args a{argc, argv};
@@ -223,18 +304,26 @@ int PreprocessGraphMain(int argc, char** argv) {
   const int format = a.i("-o", 1);
   const int device = a.i("-g", -1);
   const int threads = a.i("-t", 0);
-  if (format != 1) {
-    std::cerr << "only the .bin output format (-o 1) is written" << std::endl;
+  if (format < 0 || format > 2) {
+    std::cerr << "unknown output format " << format << " (0 .snap.txt, 1 .bin, 2 .adj.txt)" << std::endl;
     return 2;
   }
+  if (directed) throw Error("directed edge lists are not supported; pattern matching reads undirected graphs");
   auto t0 = std::chrono::steady_clock::now();
-  Graph g = ReadSNAPFile(filename, directed != 0, device, threads);
+  Graph g = LoadGraph(filename, device, threads, true);
   std::cout << "finish preprocess, time=" << Ms(t0) << "ms, now vertex_count=" << g.vertex_count << ",edge_count=" << g.cols.size()
             << std::endl;
   const std::string bad = g.Validate();
   if (!bad.empty()) throw Error("preprocessed graph is invalid: " + bad);
-  WriteCSRBinFile(filename + ".bin", g);
-  std::cout << "wrote " << filename << ".bin" << std::endl;
+  // output names of GraphIO::WriteDataFile (graph_io.h:297-384)
+  const std::string out = filename + (format == 0 ? ".snap.txt" : format == 1 ? ".bin" : ".adj.txt");
+  if (format == 0)
+    WriteSNAPText(out, g);
+  else if (format == 1)
+    WriteCSRBinFile(out, g);
+  else
+    WriteAdjText(out, g);
+  std::cout << "wrote " << out << std::endl;
   return 0;
 }
 

@@ -19,6 +19,7 @@
 
 #include "executor.h"
 #include "graph.h"
+#include "ingest.h"
 #include "kmeans_tool.h"
 #include "packing.h"
 #include "plan.h"
@@ -139,7 +140,8 @@ int Run(int argc, char** argv) {
   }
 
   auto t0 = std::chrono::steady_clock::now();
-  Graph graph = ReadCSRBinFile(filename);
+  // .bin, .txt or .lvid by suffix, as the reference's Graph(filename, false) (graph_io.h:42-55)
+  Graph graph = LoadGraph(filename, -1, thread_num, false);
   std::cout << "read graph " << filename << " V=" << graph.vertex_count << " entries=" << graph.cols.size() << " in " << Ms(t0) << " ms"
             << std::endl;
   ViewBinManager vbm(&graph, hop, thread_num);
