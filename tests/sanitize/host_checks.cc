@@ -1,0 +1,189 @@
+// Host code of libvsgm under AddressSanitizer and UndefinedBehaviorSanitizer
+// (built with g++ by tests/test_host_sanitizers.py; no GPU is touched): the
+// planner over every preset, clique and graphlet query, graph IO and
+// generators, SNAP ingestion (multi-threaded parse, large ids, text
+// writers, error paths), view-bin materialisation with reorder and split,
+// view-bin packing, and the C-ABI's host calls. Any report aborts the run
+// with a non-zero status.
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "graph.h"
+#include "ingest.h"
+#include "kmeans_tool.h"
+#include "packing.h"
+#include "plan.h"
+#include "query.h"
+#include "view.h"
+#include "vsgm.h"
+
+using namespace vsgm;
+
+namespace {
+
+int failures = 0;
+
+void Expect(bool ok, const char* what) {
+  if (!ok) {
+    std::fprintf(stderr, "check failed: %s\n", what);
+    ++failures;
+  }
+}
+
+template <class F>
+void ExpectThrow(F&& f, const char* what) {
+  try {
+    f();
+  } catch (const std::exception&) {
+    return;
+  }
+  Expect(false, what);
+}
+
+void Planner() {
+  std::vector<Query> qs;
+  for (int p = 0; p <= 18; ++p) qs.push_back(Query::Preset(p));
+  for (int k = 3; k <= 7; ++k) qs.push_back(Query::Clique(k));
+  for (int k = 3; k <= 5; ++k)
+    for (auto& q : Query::Graphlets(k)) qs.push_back(q);
+  for (auto& q : qs) {
+    Plan p(q);
+    p.Optimize();
+    Expect(!p.Serialize().empty(), "plan text");
+    Plan o = Plan::FromOrder(q, p.GetSearchSequence());
+    Expect(o.GetHop() >= 1, "FromOrder hop");
+  }
+}
+
+bool SameGraph(const Graph& a, const Graph& b) {
+  return a.vertex_count == b.vertex_count && a.row_ptrs == b.row_ptrs && a.cols == b.cols;
+}
+
+void GraphsAndIngest(const std::string& dir) {
+  Graph g = GenerateRMAT(10, 8, 0.57, 0.19, 0.19, 1, 4);
+  Expect(g.Validate().empty(), "R-MAT graph valid");
+  WriteCSRBinFile(dir + "/g.bin", g);
+  Expect(SameGraph(ReadCSRBinFile(dir + "/g.bin"), g), ".bin round trip");
+  Expect(GenerateChungLu(3000, 9000, 2.3, 2, 4).Validate().empty(), "Chung-Lu graph valid");
+
+  // SNAP text: comments, CRLF, trailing fields, a self loop, duplicates
+  std::mt19937_64 rng(5);
+  std::string text = "# comment\r\n% other\n\n";
+  for (int i = 0; i < 4000; ++i) text += std::to_string(17 + rng() % 700) + "\t" + std::to_string(17 + rng() % 700) + " 1.0\r\n";
+  text += "20 20\n";
+  std::ofstream(dir + "/g.txt") << text;
+  EdgeList ref = ReadSNAPEdges(dir + "/g.txt", 1);
+  for (int t : {2, 3, 16, 64}) Expect(ReadSNAPEdges(dir + "/g.txt", t).edges == ref.edges, "parse independent of threads");
+  Graph full = LoadGraph(dir + "/g.txt", -1, 4, false);
+  Graph kept = LoadGraph(dir + "/g.txt", -1, 4, true);
+  Expect(full.Validate().empty() && kept.Validate().empty(), "text graphs valid");
+  Expect(SameGraph(DropIsolated(full), kept), "drop isolated");
+  WriteSNAPText(dir + "/g.snap.txt", kept);
+  WriteAdjText(dir + "/g.adj.txt", kept);
+  Expect(SameGraph(LoadGraph(dir + "/g.snap.txt", -1, 2, true), kept), "SNAP text round trip");
+
+  std::ofstream(dir + "/g.lvid") << "100 2000000000000000000000\n2000000000000000000000 7\n7 100\n";
+  Graph tri = LoadGraph(dir + "/g.lvid");
+  Expect(tri.vertex_count == 3 && tri.cols.size() == 6, "large-id triangle");
+
+  std::ofstream(dir + "/tiny.txt") << "1 2\n";
+  Expect(ReadSNAPEdges(dir + "/tiny.txt", 64).edges.size() == 1, "tiny file, many threads");
+  std::ofstream(dir + "/bad.txt") << "1 2\n3\n";
+  ExpectThrow([&] { ReadSNAPEdges(dir + "/bad.txt", 2); }, "malformed line rejected");
+  std::ofstream(dir + "/odd.lvid") << "1 2 3\n";
+  ExpectThrow([&] { ReadSNAPLargeVIDEdges(dir + "/odd.lvid"); }, "odd id count rejected");
+  ExpectThrow([&] { ReadCSRBinFile(dir + "/missing.bin"); }, "missing file rejected");
+  ExpectThrow([&] { LoadGraph(dir + "/x.unknown"); }, "unknown suffix rejected");
+}
+
+void ViewsAndPacking() {
+  Graph g = GenerateRMAT(11, 8, 0.57, 0.19, 0.19, 3, 4);
+  std::mt19937 rng(7);
+  std::vector<int> part(g.vertex_count);
+  for (auto& p : part) p = (int)(rng() % 4);
+  for (int hop : {1, 2, 3}) {
+    ViewBinManager vbm(&g, hop, 4);
+    vbm.SetPartition(4, part);
+    vbm.Reorder();
+    vbm.Split(2, 2);
+    uint64_t sources = 0;
+    ViewBinHolder holder(g.vertex_count, 64ull << 20);
+    for (auto& bin : vbm.GetViewBinPool()) {
+      bin->Materialize(holder, hop == 2 ? 3 : 0);
+      sources += bin->GetSources().size();
+      Expect(holder.GetRowPtrs()[g.vertex_count] == holder.GetTotalSize(), "view row_ptrs end at the entry count");
+    }
+    Expect(sources == g.vertex_count, "bins cover every vertex");
+  }
+  const uint64_t limit = g.cols.size();
+  PackedBins one = PackViewBins(g, 2, part, 4, 0, limit, 2, true, 4);
+  std::vector<int> two_layer(g.vertex_count);
+  for (auto& p : two_layer) p = (int)(rng() % 6);
+  PackedBins two = PackViewBins(g, 2, two_layer, 2, 3, limit, 1, false, 2);
+  for (const PackedBins* p : {&one, &two}) {
+    uint64_t s = 0;
+    for (uint64_t x : p->sources) s += x;
+    Expect(s == g.vertex_count, "packed bins cover every vertex");
+  }
+  ExpectThrow([&] { PackViewBins(g, 2, part, 4, 0, 10, 1, false, 1); }, "oversized view rejected");
+  std::vector<int> a = part;
+  ReorderClusters(g, 4, a);
+  Expect(RemoveEmptyClusters(4, a) == 4, "no empty clusters");
+}
+
+// The C-ABI's host calls and their error paths; creating an executor
+// without a device must fail cleanly.
+void CApi(const std::string& dir) {
+  vsgm_graph* g = nullptr;
+  Expect(vsgm_graph_rmat(9, 8, 0.57, 0.19, 0.19, 4, 2, &g) == 0, "rmat");
+  Expect(vsgm_graph_write_bin(g, (dir + "/c.bin").c_str()) == 0, "write");
+  vsgm_graph* r = nullptr;
+  Expect(vsgm_graph_read_bin((dir + "/missing.bin").c_str(), &r) != 0 && vsgm_error()[0], "missing file reported");
+  const uint32_t bad_pairs[2] = {0, 1u << 20};
+  Expect(vsgm_graph_from_edges(10, bad_pairs, 1, &r) != 0, "out-of-range edge reported");
+  vsgm_query* q = nullptr;
+  Expect(vsgm_query_preset(2, &q) == 0, "preset");
+  vsgm_plan* p = nullptr;
+  Expect(vsgm_plan_optimize(q, &p) == 0, "optimize");
+  size_t need = 0;
+  vsgm_plan_serialize(p, nullptr, 0, &need);
+  std::vector<char> buf(need + 1);
+  Expect(vsgm_plan_serialize(p, buf.data(), buf.size(), &need) == 0, "serialize");
+  const vsgm_plan* plans[1] = {p};
+  vsgm_executor* e = nullptr;
+  int devices = 0;
+  vsgm_device_count(&devices);
+  if (devices == 0) Expect(vsgm_executor_create(0, 512, plans, 1, 512, 0, &e) != 0 && e == nullptr, "no-device executor rejected");
+  std::vector<int32_t> assign(512), part(512, 0);
+  int bins = 0;
+  Expect(vsgm_view_pack(g, 2, part.data(), 1, 0, 1ull << 40, 1, 0, 1, assign.data(), &bins, nullptr, nullptr, 0) == 0 && bins == 1,
+         "pack");
+  part[3] = 5;
+  Expect(vsgm_view_pack(g, 2, part.data(), 1, 0, 1ull << 40, 1, 0, 1, assign.data(), &bins, nullptr, nullptr, 0) != 0,
+         "bad partition id reported");
+  vsgm_plan_free(p);
+  vsgm_query_free(q);
+  vsgm_graph_free(g);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) return 2;
+  try {
+    Planner();
+    GraphsAndIngest(argv[1]);
+    ViewsAndPacking();
+    CApi(argv[1]);
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "unexpected exception: %s\n", ex.what());
+    return 1;
+  }
+  if (failures) return 1;
+  std::printf("host checks passed\n");
+  return 0;
+}
