@@ -1,7 +1,8 @@
 """CPU: the host code of libvsgm (every vsgm_amd/csrc/host/*.cc except the
 CLI mains) built with g++ under AddressSanitizer (leak detection on) and
 UndefinedBehaviorSanitizer, running tests/sanitize/host_checks.cc: planner,
-graph IO, SNAP ingestion, views, packing and the C-ABI's host calls."""
+graph IO, SNAP ingestion, views, packing and the C-ABI's host calls; and
+the pm_pipeline hand-off under ThreadSanitizer (tests/sanitize/queue_checks.cc)."""
 import glob
 import os
 import shutil
@@ -43,3 +44,18 @@ def test_host_code_is_clean_under_asan_and_ubsan(tmp_path):
     run = subprocess.run([exe, str(work)], capture_output=True, text=True, timeout=600, env=env)
     assert run.returncode == 0, (run.stdout[-2000:], run.stderr[-4000:])
     assert "host checks passed" in run.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_pipeline_hand_off_is_race_free_under_tsan(tmp_path):
+    """The producer/consumer hand-off of pm_pipeline (BoundedQueue and the
+    holder protocol, tests/sanitize/queue_checks.cc) under ThreadSanitizer,
+    including a failing consumer and close() waking a blocked push."""
+    exe = str(tmp_path / "queue_checks")
+    build = subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=thread", "-pthread", "-I" + os.path.join(AMD, "csrc", "host"),
+                            "-o", exe, os.path.join(ROOT, "tests", "sanitize", "queue_checks.cc")],
+                           capture_output=True, text=True, timeout=300)
+    assert build.returncode == 0, build.stderr[-3000:]
+    run = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=dict(os.environ, TSAN_OPTIONS="halt_on_error=1"))
+    assert run.returncode == 0, run.stderr[-4000:]
+    assert "queue checks passed" in run.stdout

@@ -17,6 +17,7 @@
 #include <thread>
 #include <vector>
 
+#include "bounded_queue.h"
 #include "executor.h"
 #include "graph.h"
 #include "ingest.h"
@@ -29,41 +30,6 @@
 
 namespace vsgm {
 namespace {
-
-// Bounded blocking queue of holder ids (the role of rigtorp::MPMCQueue in
-// pattern_matching_pipeline.cu:106-110).
-class BoundedQueue {
- public:
-  explicit BoundedQueue(size_t cap) : cap_(cap) {}
-  void push(int v) {
-    std::unique_lock<std::mutex> lk(mu_);
-    not_full_.wait(lk, [&] { return q_.size() < cap_; });
-    q_.push(v);
-    not_empty_.notify_one();
-  }
-  // Blocks until an item arrives or `closed` is set and the queue drained.
-  bool pop(int& v) {
-    std::unique_lock<std::mutex> lk(mu_);
-    not_empty_.wait(lk, [&] { return !q_.empty() || closed_; });
-    if (q_.empty()) return false;
-    v = q_.front();
-    q_.pop();
-    not_full_.notify_one();
-    return true;
-  }
-  void close() {
-    std::lock_guard<std::mutex> lk(mu_);
-    closed_ = true;
-    not_empty_.notify_all();
-  }
-
- private:
-  size_t cap_;
-  bool closed_ = false;
-  std::queue<int> q_;
-  std::mutex mu_;
-  std::condition_variable not_full_, not_empty_;
-};
 
 struct Flags {
   int argc;
