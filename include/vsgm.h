@@ -201,8 +201,17 @@ int vsgm_view_packing(int argc, char** argv);
  * -k K -i iterations -o output prefix (default <graph>.kmeans.) -d dump
  * -rc reorder -init 0, plus -g GPU and -s seed (the reference seeds with
  * the clock); writes one cluster id per line to <prefix><K>, empty clusters
- * removed. Multi-level "axb" / "a^x" clustering is rejected. */
+ * removed. Multi-level "axb" and "a^x" levels are clustered level by level;
+ * "2^x" (the reference's recursive bisection) is rejected. */
 int vsgm_kmeans_tool(int argc, char** argv);
+/* Clustering of g's vertices on GPU `device` (the tool without its file
+ * IO): levels "K", "AxBx..." or "A^X" (A != 2), KMeans::Run(std::string),
+ * kmeans.cc:253-281 and Run(std::vector<int>), :386-489 — every level
+ * splits each non-empty cluster c of the previous one into K with
+ * vsgm_kmeans_members, ids a + c * K. Writes int32[V] ids and the id range
+ * (*clusters; empty clusters are not removed). */
+int vsgm_graph_kmeans(const vsgm_graph* g, const char* levels, int iters, uint64_t seed, int device, int reorder, int32_t* assignment,
+                      int* clusters);
 /* The tool's two host steps on an assignment (int32[V], ids < k):
  * renumber clusters by decreasing count of distinct vertices in their
  * members' closed neighbourhoods (ReorderCentroids, kmeans.cc:435-452), and

@@ -51,6 +51,17 @@ int vsgm_kmeans_update(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t
 int vsgm_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, int k, int iters,
                 uint64_t seed, int32_t* h_assign, uint64_t* h_sizes, double* h_objective);
 
+/* The same run over a subset of the vertices, the step of multi-level
+ * clustering (KMeans::Run(std::vector<int>), kmeans.cc:386-489, where each
+ * level clusters the members of one previous cluster with
+ * view.SetVertices): only the d_members (u32[n_members], distinct ids <
+ * vertex_count) are clustered and make the centroids, while features stay
+ * the members' closed neighbourhoods in the whole graph. The first centroid
+ * is member SplitMix64(seed) mod n_members; k <= n_members. h_assign
+ * (int32[n_members]) follows the member order. */
+int vsgm_kmeans_members(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* d_members,
+                        uint64_t n_members, int k, int iters, uint64_t seed, int32_t* h_assign, uint64_t* h_sizes, double* h_objective);
+
 #ifdef __cplusplus
 }
 #endif

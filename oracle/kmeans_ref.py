@@ -3,7 +3,8 @@
 TEST INFRASTRUCTURE ONLY: the product runs vsgm_amd/csrc/hip/partition.hip;
 this numpy version is the checker. It follows src/tools/kmeans.cc
 (KMeansPP :69-119, UpdateAllCentroids :171-196, UpdateAssignment :198-211,
-Run :223-240) and the cosine similarity and feature scaling of
+Run :223-240, the member-subset run of each level of Run(std::vector<int>)
+:386-489) and the cosine similarity and feature scaling of
 src/tools/kmeans_utils.h:75-82, 140-150 over the closed 1-hop neighbourhoods
 of src/tools/features.cc:176-208, with the seeded first vertex and the
 (larger degree, smaller id) tie rule documented in the header. Pinned by
@@ -54,8 +55,12 @@ def update(X, a, K):
 
 
 def init(X, rp, K, first):
+    return _init(X, np.diff(rp.astype(np.int64)), np.arange(len(X)), K, first)
+
+
+def _init(X, deg, ids, K, first):
+    """Farthest-first from row `first`; ties to the larger degree, then the smaller id."""
     V = len(X)
-    deg = np.diff(rp.astype(np.int64))
     C = np.zeros((K, X.shape[1]))
     C[0] = X[first]
     maxsim = np.full(V, -np.inf)
@@ -66,7 +71,7 @@ def init(X, rp, K, first):
         maxsim = np.maximum(maxsim, s)
         m = maxsim.min()
         cand = np.nonzero(maxsim == m)[0]
-        last = int(min(cand, key=lambda v: (-deg[v], v)))
+        last = int(min(cand, key=lambda v: (-deg[v], ids[v])))
         C[k] = X[last]
     return C
 
@@ -74,6 +79,24 @@ def init(X, rp, K, first):
 def kmeans(rp, cols, K, iters, first):
     X = features(rp, cols)
     C = init(X, rp, K, first)
+    objective = []
+    for _ in range(iters + 1):
+        a, bs = assign(X, C)
+        objective.append(float(np.nansum(bs)))
+        C, sizes = update(X, a, K)
+    return a, sizes, objective
+
+
+def kmeans_members(rp, cols, K, iters, first, members):
+    """The run restricted to `members` (vsgm_kmeans_members; each level of
+    KMeans::Run(std::vector<int>), kmeans.cc:386-489): only members are
+    clustered and form centroids, features are their closed neighbourhoods
+    in the whole graph; `first` indexes `members`. Returns the members'
+    assignment in member order."""
+    members = np.asarray(members, np.int64)
+    X = features(rp, cols)[members]
+    deg = np.diff(rp.astype(np.int64))[members]
+    C = _init(X, deg, members, K, first)
     objective = []
     for _ in range(iters + 1):
         a, bs = assign(X, C)

@@ -130,6 +130,11 @@ void ViewsAndPacking() {
     Expect(s == g.vertex_count, "packed bins cover every vertex");
   }
   ExpectThrow([&] { PackViewBins(g, 2, part, 4, 0, 10, 1, false, 1); }, "oversized view rejected");
+  Expect(ParseLevels("4") == std::vector<int>{4} && ParseLevels("4x2") == std::vector<int>({4, 2}) &&
+             ParseLevels("3^2") == std::vector<int>({3, 3}),
+         "cluster level strings");
+  for (const char* bad : {"", "2^3", "4x", "x4", "17", "4y2", "99999999"})
+    ExpectThrow([&] { ParseLevels(bad); }, "bad cluster level string rejected");
   std::vector<int> a = part;
   ReorderClusters(g, 4, a);
   Expect(RemoveEmptyClusters(4, a) == 4, "no empty clusters");

@@ -117,3 +117,42 @@ def test_reference_self_test_pairs(vsgm, gpu):
         a, sizes, _ = kmeans(g, 4, iters=10, seed=seed)
         assert all(a[i] == a[i + 1] for i in range(0, 8, 2)), (seed, a)
         assert sorted(sizes.tolist()) == [2, 2, 2, 2]
+
+
+def test_member_run_matches_checker(vsgm, small):
+    """The member-subset run (each level of multi-level clustering) against
+    the checker, and with every vertex a member, equal to the whole run."""
+    from vsgm_amd.partition import first_vertex, kmeans, kmeans_members
+
+    g, rp, cl, _ = small
+    V = g.vertex_count
+    a_all, s_all, o_all = kmeans_members(g, np.arange(V), 4, iters=3, seed=5)
+    a, s, o = kmeans(g, 4, iters=3, seed=5)
+    # float64 atomics sum in any order: objectives agree to rounding, assignments up to near-ties
+    assert np.allclose(o_all, o, rtol=1e-9) and np.mean(a_all == a) > 0.99
+    members = np.nonzero(np.random.default_rng(2).random(V) < 0.4)[0]
+    for k, seed in ((2, 1), (5, 9)):
+        got_a, got_s, got_o = kmeans_members(g, members, k, iters=4, seed=seed)
+        want_a, want_s, want_o = K.kmeans_members(rp, cl, k, 4, first_vertex(seed, len(members)), members)
+        assert int(got_s.sum()) == len(members)
+        assert np.allclose(got_o, want_o, rtol=1e-9), (k, got_o, want_o)
+        assert np.mean(got_a == want_a) > 0.99, k
+    with pytest.raises(vsgm.VsgmError):
+        kmeans_members(g, np.array([0, V]), 1)
+
+
+def test_multi_level_nests_and_keeps_cliques_whole(vsgm, gpu):
+    """"3x2" on six disjoint 5-cliques: level 2 splits level 1's clusters
+    (id // 2 is the level-1 cluster), no clique is ever split, "3" equals
+    the single-level run, and "2^2" (recursive bisection) is refused."""
+    from vsgm_amd.partition import kmeans
+
+    m, n = 5, 6
+    g = vsgm.Graph.from_edges(m * n, [(c * m + i, c * m + j) for c in range(n) for i in range(m) for j in range(i + 1, m)])
+    one, k1 = vsgm.kmeans_levels(g, "3", iters=3, seed=4)
+    assert k1 == 3 and np.array_equal(one, kmeans(g, 3, iters=3, seed=4)[0])
+    two, k2 = vsgm.kmeans_levels(g, "3x2", iters=3, seed=4)
+    assert k2 == 6 and np.array_equal(two // 2, one)
+    assert all(len(set(two[c * m:(c + 1) * m].tolist())) == 1 for c in range(n))
+    with pytest.raises(vsgm.VsgmError):
+        vsgm.kmeans_levels(g, "2^2")

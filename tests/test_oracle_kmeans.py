@@ -77,3 +77,38 @@ def test_reference_self_test_pairs():
         a, sizes, _ = K.kmeans(rp, cols, 4, 10, first)
         assert all(a[i] == a[i + 1] for i in range(0, 8, 2)), (first, a)
         assert sorted(sizes.tolist()) == [2, 2, 2, 2]
+
+
+def test_member_run_equals_whole_run_and_ignores_other_rows():
+    rng = np.random.default_rng(3)
+    n = 40
+    rp, cols, adj = csr(n, [tuple(rng.integers(0, n, 2)) for _ in range(120)])
+    a, sizes, obj = K.kmeans(rp, cols, 3, 4, 5)
+    am, sm, om = K.kmeans_members(rp, cols, 3, 4, 5, np.arange(n))
+    assert np.array_equal(a, am) and np.array_equal(sizes, sm) and np.allclose(obj, om)
+    # members' features only read their own rows: rewriting the other rows changes nothing
+    members = np.arange(0, n, 2)
+    a1, s1, o1 = K.kmeans_members(rp, cols, 3, 4, 2, members)
+    lists = [list(cols[rp[v]:rp[v + 1]]) for v in range(n)]
+    for v in range(1, n, 2):
+        lists[v] = sorted(set(rng.integers(0, n, 5).tolist()) - {v})
+    rp2 = np.zeros(n + 1, np.uint64)
+    rp2[1:] = np.cumsum([len(x) for x in lists])
+    cols2 = np.array([y for x in lists for y in x], np.uint32)
+    a2, s2, o2 = K.kmeans_members(rp2, cols2, 3, 4, 2, members)
+    assert np.array_equal(a1, a2) and np.allclose(o1, o2)
+    assert int(s1.sum()) == len(members)
+
+
+def test_member_run_recovers_disjoint_cliques_among_members():
+    """Six disjoint 5-cliques; the members are the vertices of cliques 1, 3
+    and 4: k = 3 over them returns exactly those three cliques, whatever the
+    first centroid."""
+    m, n_cliques = 5, 6
+    edges = [(c * m + i, c * m + j) for c in range(n_cliques) for i in range(m) for j in range(i + 1, m)]
+    rp, cols, _ = csr(m * n_cliques, edges)
+    members = np.array([c * m + i for c in (1, 3, 4) for i in range(m)])
+    for first in range(len(members)):
+        a, sizes, _ = K.kmeans_members(rp, cols, 3, 3, first, members)
+        assert sorted(sizes.tolist()) == [m] * 3
+        assert all(len(set(a[j * m:(j + 1) * m].tolist())) == 1 for j in range(3))

@@ -112,3 +112,10 @@ def test_cluster_reorder_and_compaction(vsgm):
     assert kept == 3 and c.tolist() == [0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 2]
     with pytest.raises(vsgm.VsgmError):
         vsgm.compact_clusters(2, a)
+
+
+def test_kmeans_tool_rejects_bad_levels_before_any_work(tmp_path):
+    exe = os.path.join(ROOT, "vsgm_amd", "bin", "kmeans")
+    for levels in ("2^3", "4x", "0x", "17"):
+        out = subprocess.run([exe, "-f", str(tmp_path / "missing.bin"), "-k", levels], capture_output=True, text=True, timeout=60)
+        assert out.returncode == 2, (levels, out.stderr)

@@ -24,6 +24,7 @@ from .api import (  # noqa: F401
     kmeans_tool,
     reorder_clusters,
     compact_clusters,
+    kmeans_levels,
     library_paths,
     OPS,
 )

@@ -217,6 +217,40 @@ __global__ __launch_bounds__(kBlock) void k_pick(const double* __restrict__ d, u
   if ((threadIdx.x & (kWave - 1)) == 0 && best) atomicMax(out, (unsigned long long)best);
 }
 
+// Member subsets: mask[v] = 1 for members (v < V guaranteed by the caller's
+// id check), assignments of non-members set to -1 (they join no cluster),
+// a double array set to x outside the members, and the members' entries
+// gathered in member order.
+__global__ __launch_bounds__(kBlock) void k_km_mark(const uint32_t* __restrict__ members, uint64_t m, uint64_t V, uint8_t* __restrict__ mask,
+                                                    unsigned long long* __restrict__ bad) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t v = members[i];
+    if (v < V)
+      mask[v] = 1;
+    else
+      atomicAdd(bad, 1ull);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_km_mask_i(int32_t* __restrict__ a, const uint8_t* __restrict__ mask, uint64_t V) {
+  for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock)
+    if (!mask[v]) a[v] = -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_km_mask_d(double* __restrict__ d, const uint8_t* __restrict__ mask, uint64_t V, double x) {
+  for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock)
+    if (!mask[v]) d[v] = x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_km_gather(const int32_t* __restrict__ a, const uint32_t* __restrict__ members, uint64_t m,
+                                                      int32_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += (uint64_t)gridDim.x * kBlock) out[i] = a[members[i]];
+}
+
+__global__ void k_zero_u8(uint8_t* __restrict__ d, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) d[i] = 0;
+}
+
 __global__ void k_set_u64(unsigned long long* d, unsigned long long a, unsigned long long b) {
   d[0] = a;
   d[1] = b;
@@ -299,17 +333,19 @@ int vsgm_kmeans_update(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t
   return update_pass(ctx, d_row_ptrs, d_cols, vertex_count, k, d_assign, d_cent, d_norm, d_sizes);
 }
 
-int vsgm_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, int k, int iters,
-                uint64_t seed, int32_t* h_assign, uint64_t* h_sizes, double* h_objective) {
-  if (!ctx) return VSGM_ERR_ARG;
-  if (k < 1 || k > VSGM_KMEANS_MAX_K || iters < 0) return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "k or iterations out of range");
-  if (!d_row_ptrs || !d_cols || !h_assign || !h_sizes || vertex_count == 0 || vertex_count > 0xFFFFFFFFull)
-    return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "bad k-means arguments");
-  const uint64_t V = vertex_count;
-  const int K = k;
+}  // extern "C"
+
+namespace {
+
+// The whole run over every vertex (mask == nullptr) or over the vertices
+// with mask[v] != 0 only: non-members are never chosen as initial centroids
+// (their maxsim is pinned at the maximum), never join a cluster and add
+// nothing to the objective. `first` is the seeded first centroid. The
+// assignment stays on the device in d_assign.
+int run_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t V, int K, int iters, uint64_t first,
+               const uint8_t* mask, int32_t* assign, uint64_t* h_sizes, double* h_objective) {
   hipStream_t s = vsgm_internal::stream(ctx);
   double *cent = nullptr, *norm = nullptr, *maxsim = nullptr, *sim = nullptr, *obj = nullptr;
-  int32_t* assign = nullptr;
   uint64_t *sizes = nullptr, *keys = nullptr;
   std::vector<void*> owned;
   auto get = [&](uint64_t bytes, void** p) {
@@ -326,15 +362,15 @@ int vsgm_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_col
   if (st == VSGM_OK) st = get(V * sizeof(double), (void**)&maxsim);
   if (st == VSGM_OK) st = get(V * sizeof(double), (void**)&sim);
   if (st == VSGM_OK) st = get(sizeof(double), (void**)&obj);
-  if (st == VSGM_OK) st = get(V * sizeof(int32_t), (void**)&assign);
   if (st == VSGM_OK) st = get(K * sizeof(uint64_t), (void**)&sizes);
   if (st == VSGM_OK) st = get(2 * sizeof(uint64_t), (void**)&keys);
   if (st != VSGM_OK) return done(st);
 
-  // initialisation: farthest-first from a seeded vertex
+  // initialisation: farthest-first from the seeded vertex
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for((V * K + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cent, V * K);
   hipLaunchKernelGGL(k_fill_d, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, maxsim, V, -DBL_MAX);
-  uint64_t chosen = mix64(seed) % V;
+  if (mask) hipLaunchKernelGGL(k_km_mask_d, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, maxsim, mask, V, DBL_MAX);
+  uint64_t chosen = first;
   for (int c = 0; c < K; ++c) {
     switch (K) {
 #define X(N)                                                                                                    \
@@ -363,6 +399,10 @@ int vsgm_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_col
 
   for (int it = 0; it <= iters; ++it) {
     if ((st = assign_pass(ctx, d_row_ptrs, d_cols, V, K, cent, norm, assign, sim)) != VSGM_OK) return done(st);
+    if (mask) {
+      hipLaunchKernelGGL(k_km_mask_i, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, assign, mask, V);
+      hipLaunchKernelGGL(k_km_mask_d, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, sim, mask, V, 0.0);
+    }
     if ((st = update_pass(ctx, d_row_ptrs, d_cols, V, K, assign, cent, norm, sizes)) != VSGM_OK) return done(st);
     if (h_objective) {
       hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(kBlock), 0, s, obj, (uint64_t)1);
@@ -371,9 +411,68 @@ int vsgm_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_col
       if ((st = vsgm_d2h(ctx, &h_objective[it], obj, sizeof(double))) != VSGM_OK) return done(st);
     }
   }
-  if ((st = vsgm_d2h(ctx, h_assign, assign, V * sizeof(int32_t))) != VSGM_OK) return done(st);
   if ((st = vsgm_d2h(ctx, h_sizes, sizes, K * sizeof(uint64_t))) != VSGM_OK) return done(st);
   return done(VSGM_OK);
 }
 
+}  // namespace
+
+extern "C" {
+
+int vsgm_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, int k, int iters,
+                uint64_t seed, int32_t* h_assign, uint64_t* h_sizes, double* h_objective) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (k < 1 || k > VSGM_KMEANS_MAX_K || iters < 0) return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "k or iterations out of range");
+  if (!d_row_ptrs || !d_cols || !h_assign || !h_sizes || vertex_count == 0 || vertex_count > 0xFFFFFFFFull)
+    return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "bad k-means arguments");
+  const uint64_t V = vertex_count;
+  int32_t* assign = nullptr;
+  int st = vsgm_alloc(ctx, V * sizeof(int32_t), (void**)&assign);
+  if (st != VSGM_OK) return st;
+  st = run_kmeans(ctx, d_row_ptrs, d_cols, V, k, iters, mix64(seed) % V, nullptr, assign, h_sizes, h_objective);
+  if (st == VSGM_OK) st = vsgm_d2h(ctx, h_assign, assign, V * sizeof(int32_t));
+  vsgm_free(ctx, assign);
+  return st;
+}
+
+int vsgm_kmeans_members(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* d_members,
+                        uint64_t n_members, int k, int iters, uint64_t seed, int32_t* h_assign, uint64_t* h_sizes, double* h_objective) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (k < 1 || k > VSGM_KMEANS_MAX_K || iters < 0) return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "k or iterations out of range");
+  if (!d_row_ptrs || !d_cols || !d_members || !h_assign || !h_sizes || vertex_count == 0 || vertex_count > 0xFFFFFFFFull ||
+      n_members == 0 || n_members > vertex_count)
+    return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "bad k-means arguments");
+  if ((uint64_t)k > n_members) return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "more clusters than members");
+  const uint64_t V = vertex_count, m = n_members;
+  hipStream_t s = vsgm_internal::stream(ctx);
+  uint8_t* mask = nullptr;
+  int32_t *assign = nullptr, *out = nullptr;
+  unsigned long long* bad = nullptr;
+  auto done = [&](int st) {
+    for (void* p : {(void*)mask, (void*)assign, (void*)out, (void*)bad})
+      if (p) vsgm_free(ctx, p);
+    return st;
+  };
+  int st = vsgm_alloc(ctx, V, (void**)&mask);
+  if (st == VSGM_OK) st = vsgm_alloc(ctx, V * sizeof(int32_t), (void**)&assign);
+  if (st == VSGM_OK) st = vsgm_alloc(ctx, m * sizeof(int32_t), (void**)&out);
+  if (st == VSGM_OK) st = vsgm_alloc(ctx, 2 * sizeof(uint64_t), (void**)&bad);  // k_set_u64 writes two words
+  if (st != VSGM_OK) return done(st);
+  hipLaunchKernelGGL(k_zero_u8, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, mask, V);
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, bad, 0ull, 0ull);
+  hipLaunchKernelGGL(k_km_mark, dim3(grid_for((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, d_members, m, V, mask, bad);
+  if ((st = check(ctx, "member mask launch failed")) != VSGM_OK) return done(st);
+  // the seeded first centroid is a member; ids are checked before any use
+  uint64_t h[2] = {0, 0};
+  uint32_t first = 0;
+  if ((st = vsgm_d2h(ctx, h, bad, sizeof(h))) != VSGM_OK) return done(st);
+  if (h[0]) return done(vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "member id out of range"));
+  if ((st = vsgm_d2h(ctx, &first, d_members + mix64(seed) % m, sizeof(first))) != VSGM_OK) return done(st);
+  st = run_kmeans(ctx, d_row_ptrs, d_cols, V, k, iters, first, mask, assign, h_sizes, h_objective);
+  if (st != VSGM_OK) return done(st);
+  hipLaunchKernelGGL(k_km_gather, dim3(grid_for((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, assign, d_members, m, out);
+  if ((st = check(ctx, "member gather launch failed")) != VSGM_OK) return done(st);
+  if ((st = vsgm_d2h(ctx, h_assign, out, m * sizeof(int32_t))) != VSGM_OK) return done(st);
+  return done(VSGM_OK);
+}
 }  // extern "C"

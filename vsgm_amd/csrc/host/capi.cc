@@ -249,6 +249,15 @@ int vsgm_view_pack(const vsgm_graph* g, int hop, const int32_t* partition, int f
   });
 }
 
+int vsgm_graph_kmeans(const vsgm_graph* g, const char* levels, int iters, uint64_t seed, int device, int reorder, int32_t* assignment,
+                      int* clusters) {
+  return Guard([&] {
+    VSGM_CHECK(g && levels && assignment && clusters && iters >= 0, "bad arguments");
+    std::vector<int> a = vsgm::GpuKMeansLevels(g->g, vsgm::ParseLevels(levels), iters, seed, device, reorder != 0, clusters);
+    std::copy(a.begin(), a.end(), assignment);
+  });
+}
+
 int vsgm_clusters_reorder(const vsgm_graph* g, int k, int32_t* assignment) {
   return Guard([&] {
     VSGM_CHECK(g && assignment && k >= 1, "bad arguments");

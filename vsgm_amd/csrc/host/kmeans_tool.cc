@@ -40,11 +40,17 @@ std::vector<int> GpuKMeans(const Graph& g, int k, int iters, uint64_t seed, int 
   return std::vector<int>(assign.begin(), assign.end());
 }
 
-void ReorderClusters(const Graph& g, int k, std::vector<int>& assignment) {
+namespace {
+
+// old -> new cluster ids by decreasing count of distinct vertices in the
+// closed neighbourhoods of each cluster's members (members[j] has local
+// cluster a[j] < k); std::sort with the reference's comparator.
+std::vector<int> NonZeroOrder(const Graph& g, const std::vector<uint32_t>* members, const std::vector<int>& a, int k) {
   const uint64_t V = g.vertex_count;
   std::vector<std::vector<uint64_t>> marks(k, std::vector<uint64_t>((V + 63) / 64, 0));
-  for (uint64_t v = 0; v < V; ++v) {
-    auto& m = marks[assignment[v]];
+  for (uint64_t j = 0; j < a.size(); ++j) {
+    const uint64_t v = members ? (*members)[j] : j;
+    auto& m = marks[a[j]];
     m[v >> 6] |= 1ull << (v & 63);
     for (uint64_t i = g.row_ptrs[v]; i < g.row_ptrs[v + 1]; ++i) m[g.cols[i] >> 6] |= 1ull << (g.cols[i] & 63);
   }
@@ -53,10 +59,109 @@ void ReorderClusters(const Graph& g, int k, std::vector<int>& assignment) {
     for (uint64_t w : marks[c]) nonzero[c] += __builtin_popcountll(w);
   std::vector<unsigned> new2old(k);
   std::iota(new2old.begin(), new2old.end(), 0u);
-  std::sort(new2old.begin(), new2old.end(), [&](unsigned a, unsigned b) { return nonzero[a] > nonzero[b]; });
+  std::sort(new2old.begin(), new2old.end(), [&](unsigned x, unsigned y) { return nonzero[x] > nonzero[y]; });
   std::vector<int> old2new(k);
   for (int i = 0; i < k; ++i) old2new[new2old[i]] = i;
+  return old2new;
+}
+
+}  // namespace
+
+void ReorderClusters(const Graph& g, int k, std::vector<int>& assignment) {
+  VSGM_CHECK(assignment.size() == g.vertex_count, "assignment size does not match the graph");
+  const std::vector<int> old2new = NonZeroOrder(g, nullptr, assignment, k);
   for (auto& a : assignment) a = old2new[a];
+}
+
+std::vector<int> ParseLevels(const std::string& ks) {
+  auto number = [&](const std::string& t) {
+    VSGM_CHECK(!t.empty() && t.size() < 6 && t.find_first_not_of("0123456789") == std::string::npos, "bad cluster count in \"" + ks + "\"");
+    return std::stoi(t);
+  };
+  std::vector<int> levels;
+  const size_t caret = ks.find('^');
+  if (caret != std::string::npos) {
+    const int a = number(ks.substr(0, caret)), x = number(ks.substr(caret + 1));
+    // "2^x" is the reference's recursive bisection with centroid-ordered
+    // halves (Run2x, kmeans.cc:284-383), a different algorithm
+    VSGM_CHECK(a != 2, "recursive bisection (\"2^x\") is not supported; use \"2x2x...\"");
+    VSGM_CHECK(x >= 1 && x <= 8, "bad level count in \"" + ks + "\"");
+    levels.assign(x, a);
+  } else {
+    for (size_t start = 0;;) {
+      const size_t pos = ks.find('x', start);
+      levels.push_back(number(ks.substr(start, pos == std::string::npos ? std::string::npos : pos - start)));
+      if (pos == std::string::npos) break;
+      start = pos + 1;
+    }
+  }
+  uint64_t total = 1;
+  for (int k : levels) {
+    VSGM_CHECK(k >= 1 && k <= VSGM_KMEANS_MAX_K, "cluster counts must be in [1, 16]");
+    total *= (uint64_t)k;
+  }
+  VSGM_CHECK(total <= (1u << 30), "too many clusters");
+  return levels;
+}
+
+std::vector<int> GpuKMeansLevels(const Graph& g, const std::vector<int>& levels, int iters, uint64_t seed, int device, bool reorder,
+                                 int* clusters) {
+  const uint64_t V = g.vertex_count;
+  VSGM_CHECK(V > 0, "empty graph");
+  vsgm_ctx* ctx = nullptr;
+  if (vsgm_ctx_create(device, 0, &ctx) != VSGM_OK) throw Error("GPU k-means needs GPU " + std::to_string(device));
+  void *rp = nullptr, *cols = nullptr, *dm = nullptr;
+  auto check = [&](int s, const char* what) {
+    if (s != VSGM_OK) {
+      std::string msg = std::string(what) + " failed: " + vsgm_last_error(ctx);
+      vsgm_ctx_destroy(ctx);
+      throw Error(msg);
+    }
+  };
+  check(vsgm_alloc(ctx, (V + 1) * sizeof(uint64_t), &rp), "k-means allocation");
+  check(vsgm_alloc(ctx, std::max<size_t>(g.cols.size(), 1) * sizeof(uint32_t), &cols), "k-means allocation");
+  check(vsgm_alloc(ctx, V * sizeof(uint32_t), &dm), "k-means allocation");
+  check(vsgm_h2d(ctx, rp, g.row_ptrs.data(), (V + 1) * sizeof(uint64_t)), "graph upload");
+  check(vsgm_h2d(ctx, cols, g.cols.data(), g.cols.size() * sizeof(uint32_t)), "graph upload");
+  // KMeans::Run(std::vector<int>) (kmeans.cc:386-489): every level splits
+  // each non-empty cluster of the previous one into K, ids a + c * K
+  std::vector<int> global(V, 0);
+  std::vector<uint64_t> sizes(1, V);
+  for (size_t level = 0; level < levels.size(); ++level) {
+    const int K = levels[level];
+    if (K <= 1) continue;
+    std::vector<std::vector<uint32_t>> parts(sizes.size());
+    for (uint64_t v = 0; v < V; ++v) parts[global[v]].push_back((uint32_t)v);
+    parts.erase(std::remove_if(parts.begin(), parts.end(), [](const std::vector<uint32_t>& p) { return p.empty(); }), parts.end());
+    std::vector<uint64_t> next_sizes(parts.size() * K, 0);
+    for (size_t c = 0; c < parts.size(); ++c) {
+      const auto& part = parts[c];
+      const uint64_t m = part.size();
+      const int k = (int)std::min<uint64_t>(K, m);
+      std::vector<int32_t> a(m);
+      std::vector<uint64_t> sz(k, 0);
+      if (m == V) {
+        // the whole graph: the single-level run, same seed
+        check(vsgm_kmeans(ctx, (const uint64_t*)rp, (const uint32_t*)cols, V, k, iters, seed, a.data(), sz.data(), nullptr), "k-means");
+      } else {
+        const uint64_t s = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)level * 65536 + c + 1);
+        check(vsgm_h2d(ctx, dm, part.data(), m * sizeof(uint32_t)), "member upload");
+        check(vsgm_kmeans_members(ctx, (const uint64_t*)rp, (const uint32_t*)cols, V, (const uint32_t*)dm, m, k, iters, s, a.data(),
+                                  sz.data(), nullptr),
+              "k-means");
+      }
+      std::vector<int> local(a.begin(), a.end());
+      std::vector<int> old2new(k);
+      std::iota(old2new.begin(), old2new.end(), 0);
+      if (reorder) old2new = NonZeroOrder(g, m == V ? nullptr : &part, local, k);
+      for (uint64_t j = 0; j < m; ++j) global[part[j]] = old2new[local[j]] + (int)c * K;
+      for (int x = 0; x < k; ++x) next_sizes[old2new[x] + c * K] = sz[x];
+    }
+    sizes.swap(next_sizes);
+  }
+  vsgm_ctx_destroy(ctx);  // frees the pool
+  if (clusters) *clusters = (int)sizes.size();
+  return global;
 }
 
 int RemoveEmptyClusters(int k, std::vector<int>& assignment) {
@@ -98,19 +203,21 @@ int KMeansToolMain(int argc, char** argv) {
   const int init = (int)num("-init", 0);
   const int device = (int)num("-g", 0);
   const uint64_t seed = (uint64_t)num("-s", 1);
-  if (ks.empty() || ks.find_first_not_of("0123456789") != std::string::npos) {
-    std::cerr << "multi-level clustering (\"axb\", \"a^x\") is not supported; give one cluster count" << std::endl;
+  std::vector<int> levels;
+  try {
+    levels = ParseLevels(ks);
+  } catch (const std::exception& ex) {
+    std::cerr << ex.what() << std::endl;
     return 2;
   }
   if (init != 0) {
     std::cerr << "only the farthest-first initialisation (-init 0) is supported" << std::endl;
     return 2;
   }
-  const int k = std::stoi(ks);
   Graph g = ReadCSRBinFile(filename);
   auto t0 = std::chrono::steady_clock::now();
-  std::vector<int> assignment = GpuKMeans(g, k, iters, seed, device);
-  if (reorder) ReorderClusters(g, k, assignment);
+  int k = 0;
+  std::vector<int> assignment = GpuKMeansLevels(g, levels, iters, seed, device, reorder != 0, &k);
   const int kept = RemoveEmptyClusters(k, assignment);
   std::cout << "kmeans: " << kept << " non-empty clusters of " << k << " in "
             << std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() << " ms" << std::endl;

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "graph.h"
@@ -15,6 +16,15 @@ std::vector<int> GpuKMeans(const Graph& g, int k, int iters, uint64_t seed, int 
 // members' closed neighbourhoods, i.e. non-zero centroid entries
 // (ReorderCentroids, kmeans.cc:435-452).
 void ReorderClusters(const Graph& g, int k, std::vector<int>& assignment);
+// "a", "axbx..." (one level per factor) or "a^x" (x levels of a, a != 2)
+// into per-level cluster counts (KMeans::Run(std::string), kmeans.cc:253-281).
+std::vector<int> ParseLevels(const std::string& ks);
+// Multi-level clustering (KMeans::Run(std::vector<int>), kmeans.cc:386-489):
+// each level splits every non-empty cluster c of the previous level into K
+// with the member-subset k-means, ids a + c * K; reorder renumbers each
+// split by centroid non-zeros. *clusters = id range of the result.
+std::vector<int> GpuKMeansLevels(const Graph& g, const std::vector<int>& levels, int iters, uint64_t seed, int device, bool reorder,
+                                 int* clusters);
 // Empty clusters removed and the rest renumbered in order
 // (RemoveEmptyClusters, kmeans.cc:491-510); returns the cluster count.
 int RemoveEmptyClusters(int k, std::vector<int>& assignment);

@@ -21,6 +21,7 @@ def _lib():
             "vsgm_kmeans_assign": ([P, P, P, u64, i, P, P, P, P], i),
             "vsgm_kmeans_update": ([P, P, P, u64, i, P, P, P, P], i),
             "vsgm_kmeans": ([P, P, P, u64, i, i, u64, P, P, P], i),
+            "vsgm_kmeans_members": ([P, P, P, u64, P, u64, i, i, u64, P, P, P], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -59,6 +60,22 @@ def kmeans(graph: Graph, k: int, iters: int = 10, seed: int = 1, ctx: Optional[D
     obj = np.zeros(iters + 1, np.float64)
     dg.ctx.check(_lib().vsgm_kmeans(dg.ctx.h, dg.rp.ptr, dg.cols.ptr, dg.V, k, iters, seed, a.ctypes.data, sizes.ctypes.data,
                                     obj.ctypes.data), "kmeans")
+    return a, sizes, obj
+
+
+def kmeans_members(graph: Graph, members, k: int, iters: int = 10, seed: int = 1, ctx: Optional[DeviceContext] = None):
+    """The run restricted to `members` (features over the whole graph);
+    returns (assignment int32[len(members)] in member order, sizes, objective)."""
+    if not 1 <= k <= MAX_K:
+        raise VsgmError(f"k must be in [1, {MAX_K}]")
+    dg = DeviceGraph(graph, ctx)
+    mem = np.ascontiguousarray(members, dtype=np.uint32)
+    dmem = dg.ctx.array(mem)
+    a = np.zeros(mem.size, np.int32)
+    sizes = np.zeros(k, np.uint64)
+    obj = np.zeros(iters + 1, np.float64)
+    dg.ctx.check(_lib().vsgm_kmeans_members(dg.ctx.h, dg.rp.ptr, dg.cols.ptr, dg.V, dmem.ptr, mem.size, k, iters, seed, a.ctypes.data,
+                                            sizes.ctypes.data, obj.ctypes.data), "kmeans_members")
     return a, sizes, obj
 
 
