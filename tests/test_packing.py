@@ -74,7 +74,7 @@ def test_live_against_reference_tool(vsgm, tmp_path, seed):
         r = run_tool(tool, gf, pf, first, hop, mem, pool, order)
         base = f"{gf}.{hop}.hop"
         outs.append((r.returncode == 0, open(base + ".vbmap").read() if r.returncode == 0 else None,
-                     open(base + ".vbstat").read() if r.returncode == 0 else None))
+                     open(base + ".vbstat").read() if r.returncode == 0 else None, open(gf + ".degrees.bin", "rb").read()))
     bins = outs[0][1].split()[0] if outs[0][0] else "-"
     print(f"case scale={scale} ef={ef} layers={first}x{second} hop={hop} pool={pool} order={order}: bins={bins}")
     assert outs[0] == outs[1]
@@ -119,3 +119,14 @@ def test_kmeans_tool_rejects_bad_levels_before_any_work(tmp_path):
     for levels in ("2^3", "4x", "0x", "17"):
         out = subprocess.run([exe, "-f", str(tmp_path / "missing.bin"), "-k", levels], capture_output=True, text=True, timeout=60)
         assert out.returncode == 2, (levels, out.stderr)
+
+
+def test_degree_cache_must_match_the_graph(vsgm, tmp_path):
+    g, gf, pf = setup_case(vsgm, str(tmp_path), 9, 8, 1, np.zeros(512, int), 1, 0)
+    assert run_tool(TOOL, gf, pf, 1, 2, 1.0, 1, 0).returncode == 0
+    raw = np.fromfile(gf + ".degrees.bin", dtype=np.uint64)
+    rp, _ = g.csr()
+    assert raw[0] == g.vertex_count and np.array_equal(raw[1:], np.diff(rp))
+    raw[5] += 1
+    raw.tofile(gf + ".degrees.bin")
+    assert run_tool(TOOL, gf, pf, 1, 2, 1.0, 1, 0).returncode == 1

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -284,6 +285,27 @@ int ViewPackingMain(int argc, char** argv) {
   const uint64_t max_entries = (uint64_t)((mem * 1000 * 1000 * 1000) / sizeof(uintV));
   std::cout << "mem: " << mem << "G, max view bin size: " << max_entries << std::endl;
   Graph g = ReadCSRBinFile(graph_file);
+  // the degree cache beside the graph (view_packing.cc:40-114): written when
+  // absent, checked against the graph when present
+  const std::string degrees_file = graph_file + ".degrees.bin";
+  if (FILE* f = fopen(degrees_file.c_str(), "rb")) {
+    uint64_t n = 0;
+    bool ok = fread(&n, sizeof(n), 1, f) == 1 && n == g.vertex_count;
+    for (uint64_t v = 0; ok && v < n; ++v) {
+      uint64_t d = 0;
+      ok = fread(&d, sizeof(d), 1, f) == 1 && d == g.Degree((uintV)v);
+    }
+    fclose(f);
+    if (!ok) throw Error(degrees_file + " does not match " + graph_file);
+  } else {
+    std::vector<uint64_t> out(g.vertex_count + 1);
+    out[0] = g.vertex_count;
+    for (uint64_t v = 0; v < g.vertex_count; ++v) out[v + 1] = g.Degree((uintV)v);
+    FILE* w = fopen(degrees_file.c_str(), "wb");
+    if (!w) throw Error("cannot write " + degrees_file);
+    const bool ok = fwrite(out.data(), sizeof(uint64_t), out.size(), w) == out.size();
+    if (fclose(w) != 0 || !ok) throw Error("write to " + degrees_file + " failed");
+  }
   auto t0 = std::chrono::steady_clock::now();
   PackedBins p = PackViewBins(g, hop, ReadPartitionFile(part_file, g.vertex_count), first, second, max_entries, pool_size, degree_order != 0);
   std::cout << "total " << p.bins << " bins after merging (" << Ms(t0) << " ms)" << std::endl;
