@@ -204,6 +204,11 @@ int vsgm_view_packing(int argc, char** argv);
  * removed. Multi-level "axb" and "a^x" levels are clustered level by level;
  * "2^x" (the reference's recursive bisection) is rejected. */
 int vsgm_kmeans_tool(int argc, char** argv);
+/* The features tool (src/tools/features.cc:68-207) for -h 1: -f graph writes
+ * <graph>.1hop.bin, every row plus its own vertex, sorted. Other -h and
+ * -dn are rejected. The kmeans tool reads such a file as the reference's
+ * does (rows that hold their own vertex are clustered as they are). */
+int vsgm_features_tool(int argc, char** argv);
 /* Clustering of g's vertices on GPU `device` (the tool without its file
  * IO): levels "K", "AxBx..." or "A^X" (A != 2), KMeans::Run(std::string),
  * kmeans.cc:253-281 and Run(std::vector<int>), :386-489 — every level

@@ -135,6 +135,12 @@ void ViewsAndPacking() {
          "cluster level strings");
   for (const char* bad : {"", "2^3", "4x", "x4", "17", "4y2", "99999999"})
     ExpectThrow([&] { ParseLevels(bad); }, "bad cluster level string rejected");
+  Graph closed = ClosedNeighbourhoods(g);
+  Expect(closed.cols.size() == g.cols.size() + g.vertex_count, "closed rows add one entry per vertex");
+  Graph opened = closed;
+  Expect(OpenIfClosed(opened) && SameGraph(opened, g), "opening closed rows restores the graph");
+  Graph plain = g;
+  Expect(!OpenIfClosed(plain) && SameGraph(plain, g), "open rows stay as they are");
   std::vector<int> a = part;
   ReorderClusters(g, 4, a);
   Expect(RemoveEmptyClusters(4, a) == 4, "no empty clusters");

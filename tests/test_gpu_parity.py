@@ -266,3 +266,22 @@ def test_offline_tool_chain(vsgm, oracle, gpu, tmp_path):
     out = run("pm_pipeline", "-f", path + ".bin", "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4", "-kc", "4")
     line = [x for x in out.splitlines() if x.startswith("total count for query 0:")][0]
     assert int(line.split(":")[1]) == oracle_counts(oracle, g, [vsgm.Plan.optimize(vsgm.Query.clique(4))])[0]
+
+
+def test_kmeans_tool_reads_features_graphs_as_the_reference_does(vsgm, gpu, tmp_path):
+    """`features -h 1` then `kmeans -f G.bin.1hop.bin` (the reference's
+    documented steps) gives the partition `kmeans -f G.bin` gives: both
+    cluster the closed rows."""
+    g = vsgm.Graph.rmat(10, 8, seed=31)
+    path = str(tmp_path / "g.bin")
+    g.write_bin(path)
+    bindir = os.path.join(ROOT, "vsgm_amd", "bin")
+    for args in (["features", "-f", path, "-h", "1"], ["kmeans", "-f", path, "-k", "4", "-i", "4"],
+                 ["kmeans", "-f", path + ".1hop.bin", "-k", "4", "-i", "4"]):
+        print("[gpu]", *args, flush=True)
+        out = subprocess.run([os.path.join(bindir, args[0])] + args[1:], capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr[-2000:]
+    plain = np.loadtxt(path + ".kmeans.4", dtype=np.int64)
+    closed = np.loadtxt(path + ".1hop.bin.kmeans.4", dtype=np.int64)
+    # float64 atomics sum in any order: equal up to near-ties
+    assert len(plain) == g.vertex_count and np.mean(plain == closed) > 0.99

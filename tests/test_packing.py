@@ -130,3 +130,31 @@ def test_degree_cache_must_match_the_graph(vsgm, tmp_path):
     raw[5] += 1
     raw.tofile(gf + ".degrees.bin")
     assert run_tool(TOOL, gf, pf, 1, 2, 1.0, 1, 0).returncode == 1
+
+
+REF_FEATURES = os.path.join(ROOT, "oracle", "_ref", "features")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_FEATURES), reason="oracle/_ref/features is built only where the reference tree exists")
+@pytest.mark.parametrize("scale,ef,seed", [(9, 8, 1), (11, 4, 2)])
+def test_features_tool_matches_reference_tool(vsgm, tmp_path, scale, ef, seed):
+    g = vsgm.Graph.rmat(scale, ef, seed=seed)
+    outs = []
+    for name, tool in (("ref", REF_FEATURES), ("me", os.path.join(ROOT, "vsgm_amd", "bin", "features"))):
+        d = tmp_path / name
+        d.mkdir()
+        g.write_bin(str(d / "g.bin"))
+        r = subprocess.run([tool, "-f", str(d / "g.bin"), "-h", "1", "-t", "4"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append((d / "g.bin.1hop.bin").read_bytes())
+    assert outs[0] == outs[1]
+
+
+def test_features_tool_writes_closed_rows(vsgm, tmp_path):
+    g = vsgm.Graph.from_edges(5, [(0, 1), (1, 2), (3, 4)])
+    g.write_bin(str(tmp_path / "g.bin"))
+    exe = os.path.join(ROOT, "vsgm_amd", "bin", "features")
+    assert subprocess.run([exe, "-f", str(tmp_path / "g.bin"), "-h", "1"], capture_output=True, timeout=60).returncode == 0
+    rp, cols = vsgm.Graph.read_bin(str(tmp_path / "g.bin.1hop.bin")).csr()
+    assert rp.tolist() == [0, 2, 5, 7, 9, 11] and cols.tolist() == [0, 1, 0, 1, 2, 1, 2, 3, 4, 3, 4]
+    assert subprocess.run([exe, "-f", str(tmp_path / "g.bin"), "-h", "2"], capture_output=True, timeout=60).returncode == 2

@@ -12,6 +12,7 @@
 #include <numeric>
 #include <string>
 
+#include "ingest.h"
 #include "vsgm_hip.h"
 #include "vsgm_partition.h"
 
@@ -71,6 +72,43 @@ void ReorderClusters(const Graph& g, int k, std::vector<int>& assignment) {
   VSGM_CHECK(assignment.size() == g.vertex_count, "assignment size does not match the graph");
   const std::vector<int> old2new = NonZeroOrder(g, nullptr, assignment, k);
   for (auto& a : assignment) a = old2new[a];
+}
+
+Graph ClosedNeighbourhoods(const Graph& g) {
+  Graph out;
+  out.vertex_count = g.vertex_count;
+  out.row_ptrs.resize(g.vertex_count + 1);
+  out.cols.resize(g.cols.size() + g.vertex_count);
+  for (uint64_t v = 0; v <= g.vertex_count; ++v) out.row_ptrs[v] = g.row_ptrs[v] + v;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (uint64_t v = 0; v < g.vertex_count; ++v) {
+    auto b = out.cols.begin() + out.row_ptrs[v];
+    std::copy(g.cols.begin() + g.row_ptrs[v], g.cols.begin() + g.row_ptrs[v + 1], b);
+    b[g.row_ptrs[v + 1] - g.row_ptrs[v]] = (uint32_t)v;
+    std::sort(b, out.cols.begin() + out.row_ptrs[v + 1]);
+  }
+  return out;
+}
+
+bool OpenIfClosed(Graph& g) {
+  for (uint64_t v = 0; v < g.vertex_count; ++v)
+    if (!std::binary_search(g.cols.begin() + g.row_ptrs[v], g.cols.begin() + g.row_ptrs[v + 1], (uint32_t)v)) return false;
+  uint64_t w = 0;
+  for (uint64_t v = 0; v < g.vertex_count; ++v) {
+    const uint64_t b = g.row_ptrs[v], e = g.row_ptrs[v + 1];
+    bool dropped = false;
+    g.row_ptrs[v] = w;
+    for (uint64_t i = b; i < e; ++i) {
+      if (!dropped && g.cols[i] == v) {
+        dropped = true;
+        continue;
+      }
+      g.cols[w++] = g.cols[i];
+    }
+  }
+  g.row_ptrs[g.vertex_count] = w;
+  g.cols.resize(w);
+  return true;
 }
 
 std::vector<int> ParseLevels(const std::string& ks) {
@@ -215,6 +253,10 @@ int KMeansToolMain(int argc, char** argv) {
     return 2;
   }
   Graph g = ReadCSRBinFile(filename);
+  // The reference clusters the rows as given and is run on the closed rows
+  // written by `features -h 1`; the GPU k-means closes every row itself,
+  // so rows that already hold their own vertex are opened first.
+  if (OpenIfClosed(g)) std::cout << "kmeans: rows hold their own vertex (a features -h 1 graph); clustering their closed rows" << std::endl;
   auto t0 = std::chrono::steady_clock::now();
   int k = 0;
   std::vector<int> assignment = GpuKMeansLevels(g, levels, iters, seed, device, reorder != 0, &k);
@@ -236,6 +278,37 @@ extern "C" int vsgm_kmeans_tool(int argc, char** argv) {
     return vsgm::KMeansToolMain(argc, argv);
   } catch (const std::exception& ex) {
     std::cerr << "kmeans: " << ex.what() << std::endl;
+    return 1;
+  }
+}
+
+namespace vsgm {
+
+int FeaturesToolMain(int argc, char** argv) {
+  if (argc == 1) return -1;
+  auto str = [&](const char* key, const std::string& d) { return Flag(argc, argv, key) ? std::string(Flag(argc, argv, key)) : d; };
+  auto num = [&](const char* key, long d) { return Flag(argc, argv, key) ? atol(Flag(argc, argv, key)) : d; };
+  const std::string filename = str("-f", "./data/com-dblp.ungraph.txt");
+  const int hop = (int)num("-h", 2);
+  const int duplicates = (int)num("-dn", 0);
+  if (hop != 1 || duplicates != 0) {
+    std::cerr << "only 1-hop features (-h 1, no -dn) are written: the GPU k-means reads the closed rows they hold" << std::endl;
+    return 2;
+  }
+  Graph g = LoadGraph(filename, -1, 0, false);
+  // "<file>.1hop.bin" (features.cc:176-207): each row plus its own vertex, sorted
+  WriteCSRBinFile(filename + ".1hop.bin", ClosedNeighbourhoods(g));
+  std::cout << "wrote " << filename << ".1hop.bin" << std::endl;
+  return 0;
+}
+
+}  // namespace vsgm
+
+extern "C" int vsgm_features_tool(int argc, char** argv) {
+  try {
+    return vsgm::FeaturesToolMain(argc, argv);
+  } catch (const std::exception& ex) {
+    std::cerr << "features: " << ex.what() << std::endl;
     return 1;
   }
 }

@@ -16,6 +16,12 @@ std::vector<int> GpuKMeans(const Graph& g, int k, int iters, uint64_t seed, int 
 // members' closed neighbourhoods, i.e. non-zero centroid entries
 // (ReorderCentroids, kmeans.cc:435-452).
 void ReorderClusters(const Graph& g, int k, std::vector<int>& assignment);
+// Each row plus its own vertex, sorted: the ".1hop" graph of the features
+// tool (features.cc:176-207).
+Graph ClosedNeighbourhoods(const Graph& g);
+// When every row holds its own vertex, remove it once from each row (the
+// inverse of ClosedNeighbourhoods) and return true; otherwise leave g alone.
+bool OpenIfClosed(Graph& g);
 // "a", "axbx..." (one level per factor) or "a^x" (x levels of a, a != 2)
 // into per-level cluster counts (KMeans::Run(std::string), kmeans.cc:253-281).
 std::vector<int> ParseLevels(const std::string& ks);
