@@ -209,6 +209,12 @@ int vsgm_kmeans_tool(int argc, char** argv);
  * -dn are rejected. The kmeans tool reads such a file as the reference's
  * does (rows that hold their own vertex are clustered as they are). */
 int vsgm_features_tool(int argc, char** argv);
+/* The fennel tool (src/tools/fennel.cc:21-101): -f graph -p parts -h 1|2;
+ * writes <graph>.fennel.part.<p>, one part per vertex, the same parts as
+ * the reference's streaming partitioner. vsgm_graph_fennel is the same
+ * partition as a library call (int32[V]). */
+int vsgm_fennel_tool(int argc, char** argv);
+int vsgm_graph_fennel(const vsgm_graph* g, int parts, int hop, int32_t* assignment);
 /* Clustering of g's vertices on GPU `device` (the tool without its file
  * IO): levels "K", "AxBx..." or "A^X" (A != 2), KMeans::Run(std::string),
  * kmeans.cc:253-281 and Run(std::vector<int>), :386-489 — every level

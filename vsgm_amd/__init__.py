@@ -25,6 +25,7 @@ from .api import (  # noqa: F401
     reorder_clusters,
     compact_clusters,
     kmeans_levels,
+    fennel,
     library_paths,
     OPS,
 )

@@ -148,6 +148,7 @@ def _declare(L):
         "vsgm_view_packing": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
         "vsgm_kmeans_tool": ([C.c_int, C.POINTER(C.c_char_p)], C.c_int),
         "vsgm_clusters_reorder": ([P, C.c_int, P], C.c_int),
+        "vsgm_graph_fennel": ([P, C.c_int, C.c_int, P], C.c_int),
         "vsgm_graph_kmeans": ([P, C.c_char_p, C.c_int, u64, C.c_int, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
         "vsgm_clusters_compact": ([C.c_int, P, u64, C.POINTER(C.c_int)], C.c_int),
         "vsgm_view_pack": ([P, C.c_int, P, C.c_int, C.c_int, u64, C.c_int, C.c_int, C.c_int, P, C.POINTER(C.c_int), P, P, C.c_int],
@@ -683,3 +684,10 @@ def kmeans_levels(graph: "Graph", levels: str, iters: int = 10, seed: int = 1, d
     n = C.c_int()
     _check(lib().vsgm_graph_kmeans(graph._h, levels.encode(), iters, seed, device, int(reorder), _ptr(a), C.byref(n)), "graph_kmeans")
     return a, n.value
+
+
+def fennel(graph: "Graph", parts: int, hop: int = 1) -> np.ndarray:
+    """Streaming Fennel partition (the reference's fennel tool); int32[V]."""
+    a = np.zeros(graph.vertex_count, np.int32)
+    _check(lib().vsgm_graph_fennel(graph._h, parts, hop, _ptr(a)), "graph_fennel")
+    return a

@@ -258,6 +258,14 @@ int vsgm_graph_kmeans(const vsgm_graph* g, const char* levels, int iters, uint64
   });
 }
 
+int vsgm_graph_fennel(const vsgm_graph* g, int parts, int hop, int32_t* assignment) {
+  return Guard([&] {
+    VSGM_CHECK(g && assignment, "bad arguments");
+    std::vector<int> a = vsgm::FennelPartition(g->g, parts, hop);
+    std::copy(a.begin(), a.end(), assignment);
+  });
+}
+
 int vsgm_clusters_reorder(const vsgm_graph* g, int k, int32_t* assignment) {
   return Guard([&] {
     VSGM_CHECK(g && assignment && k >= 1, "bad arguments");

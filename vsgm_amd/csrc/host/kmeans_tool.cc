@@ -4,6 +4,8 @@
 #include "kmeans_tool.h"
 
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -309,6 +311,82 @@ extern "C" int vsgm_features_tool(int argc, char** argv) {
     return vsgm::FeaturesToolMain(argc, argv);
   } catch (const std::exception& ex) {
     std::cerr << "features: " << ex.what() << std::endl;
+    return 1;
+  }
+}
+
+namespace vsgm {
+
+std::vector<int> FennelPartition(const Graph& g, int parts, int hop) {
+  VSGM_CHECK(parts >= 1 && (hop == 1 || hop == 2), "fennel needs parts >= 1 and hop 1 or 2");
+  // The reference's constants and arithmetic types (fennel.cc:40-44):
+  // alpha and miu are floats, the balance term is double, scores are floats.
+  const size_t vertex_num = g.vertex_count;
+  const size_t edge_num = g.cols.size() / 2;
+  const float gamma = 1.5;
+  const float alpha = std::sqrt(parts) * edge_num / std::pow(vertex_num, gamma);
+  const float slack = 1.1;
+  const float miu = slack * vertex_num / parts;
+  std::vector<int> part(vertex_num, -1);
+  std::vector<long> size(parts, 0);
+  // Streaming assignment in vertex order (fennel.cc:54-89). A neighbour
+  // counts for part i when it was already placed there; the reference's
+  // union-find holds exactly that relation (each placed vertex points at
+  // its part's first vertex).
+  auto in_part = [&](uintV u, int i) { return part[u] == i; };
+  for (uint64_t v = 0; v < vertex_num; ++v) {
+    float max_score = -FLT_MAX;
+    int max_part = 0;
+    for (int i = 0; i < parts; ++i) {
+      if (!(size[i] <= miu)) continue;
+      const double delta_c = alpha * (std::pow(size[i] + 1, gamma) - std::pow(size[i], gamma));
+      float score = 0;
+      for (uint64_t j = g.row_ptrs[v]; j < g.row_ptrs[v + 1]; ++j) {
+        const uintV nid = g.cols[j];
+        if (in_part(nid, i)) score += 1;
+        if (hop == 2)
+          for (uint64_t k = g.row_ptrs[nid]; k < g.row_ptrs[nid + 1]; ++k)
+            if (in_part(g.cols[k], i)) score += 1;
+      }
+      score = score - delta_c;
+      if (max_score < score) {
+        max_score = score;
+        max_part = i;
+      }
+    }
+    size[max_part] += 1;
+    part[v] = max_part;
+  }
+  return part;
+}
+
+int FennelToolMain(int argc, char** argv) {
+  if (argc == 1) return -1;
+  auto str = [&](const char* key, const std::string& d) { return Flag(argc, argv, key) ? std::string(Flag(argc, argv, key)) : d; };
+  auto num = [&](const char* key, long d) { return Flag(argc, argv, key) ? atol(Flag(argc, argv, key)) : d; };
+  const std::string filename = str("-f", "./data/com-dblp.ungraph.txt");
+  const int parts = (int)num("-p", 4);
+  const int hop = (int)num("-h", 1);
+  Graph g = LoadGraph(filename, -1, 0, false);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> part = FennelPartition(g, parts, hop);
+  std::cout << "fennel: " << parts << " parts in "
+            << std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() << " ms" << std::endl;
+  // "<file>.fennel.part.<p>", one part per line (fennel.cc:94-99)
+  const std::string out = filename + ".fennel.part." + std::to_string(parts);
+  std::ofstream os(out);
+  for (int p : part) os << p << "\n";
+  if (!os) throw Error("write to " + out + " failed");
+  return 0;
+}
+
+}  // namespace vsgm
+
+extern "C" int vsgm_fennel_tool(int argc, char** argv) {
+  try {
+    return vsgm::FennelToolMain(argc, argv);
+  } catch (const std::exception& ex) {
+    std::cerr << "fennel: " << ex.what() << std::endl;
     return 1;
   }
 }

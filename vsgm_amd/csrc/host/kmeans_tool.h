@@ -31,6 +31,12 @@ std::vector<int> ParseLevels(const std::string& ks);
 // split by centroid non-zeros. *clusters = id range of the result.
 std::vector<int> GpuKMeansLevels(const Graph& g, const std::vector<int>& levels, int iters, uint64_t seed, int device, bool reorder,
                                  int* clusters);
+// Streaming Fennel partition in vertex order (src/tools/fennel.cc:21-92):
+// each vertex joins the part with the highest score (neighbours already
+// there, or with hop 2 also their neighbours, minus a balance term), parts
+// above 1.1 V / parts skipped; the reference's float and double arithmetic
+// is kept so the same graph gives the same parts.
+std::vector<int> FennelPartition(const Graph& g, int parts, int hop);
 // Empty clusters removed and the rest renumbered in order
 // (RemoveEmptyClusters, kmeans.cc:491-510); returns the cluster count.
 int RemoveEmptyClusters(int k, std::vector<int>& assignment);
