@@ -204,9 +204,10 @@ int vsgm_view_packing(int argc, char** argv);
  * removed. Multi-level "axb" and "a^x" levels are clustered level by level;
  * "2^x" (the reference's recursive bisection) is rejected. */
 int vsgm_kmeans_tool(int argc, char** argv);
-/* The features tool (src/tools/features.cc:68-207) for -h 1: -f graph writes
- * <graph>.1hop.bin, every row plus its own vertex, sorted. Other -h and
- * -dn are rejected. The kmeans tool reads such a file as the reference's
+/* The features tool (src/tools/features.cc:68-258): -f graph -h 1 writes
+ * <graph>.1hop.bin (every row plus its own vertex), -h 2 <graph>.2hop.bin
+ * (every vertex within two steps), rows sorted. The sampled -h -2 and -dn
+ * duplication are rejected. The kmeans tool reads such a file as the reference's
  * does (rows that hold their own vertex are clustered as they are). */
 int vsgm_features_tool(int argc, char** argv);
 /* The fennel tool (src/tools/fennel.cc:21-101): -f graph -p parts -h 1|2;

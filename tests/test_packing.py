@@ -136,17 +136,17 @@ REF_FEATURES = os.path.join(ROOT, "oracle", "_ref", "features")
 
 
 @pytest.mark.skipif(not os.path.exists(REF_FEATURES), reason="oracle/_ref/features is built only where the reference tree exists")
-@pytest.mark.parametrize("scale,ef,seed", [(9, 8, 1), (11, 4, 2)])
-def test_features_tool_matches_reference_tool(vsgm, tmp_path, scale, ef, seed):
+@pytest.mark.parametrize("scale,ef,seed,hop", [(9, 8, 1, 1), (11, 4, 2, 1), (9, 8, 3, 2), (10, 4, 4, 2)])
+def test_features_tool_matches_reference_tool(vsgm, tmp_path, scale, ef, seed, hop):
     g = vsgm.Graph.rmat(scale, ef, seed=seed)
     outs = []
     for name, tool in (("ref", REF_FEATURES), ("me", os.path.join(ROOT, "vsgm_amd", "bin", "features"))):
         d = tmp_path / name
         d.mkdir()
         g.write_bin(str(d / "g.bin"))
-        r = subprocess.run([tool, "-f", str(d / "g.bin"), "-h", "1", "-t", "4"], capture_output=True, text=True, timeout=120)
+        r = subprocess.run([tool, "-f", str(d / "g.bin"), "-h", str(hop), "-t", "4"], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
-        outs.append((d / "g.bin.1hop.bin").read_bytes())
+        outs.append((d / f"g.bin.{hop}hop.bin").read_bytes())
     assert outs[0] == outs[1]
 
 
@@ -157,7 +157,10 @@ def test_features_tool_writes_closed_rows(vsgm, tmp_path):
     assert subprocess.run([exe, "-f", str(tmp_path / "g.bin"), "-h", "1"], capture_output=True, timeout=60).returncode == 0
     rp, cols = vsgm.Graph.read_bin(str(tmp_path / "g.bin.1hop.bin")).csr()
     assert rp.tolist() == [0, 2, 5, 7, 9, 11] and cols.tolist() == [0, 1, 0, 1, 2, 1, 2, 3, 4, 3, 4]
-    assert subprocess.run([exe, "-f", str(tmp_path / "g.bin"), "-h", "2"], capture_output=True, timeout=60).returncode == 2
+    assert subprocess.run([exe, "-f", str(tmp_path / "g.bin"), "-h", "2"], capture_output=True, timeout=60).returncode == 0
+    rp, cols = vsgm.Graph.read_bin(str(tmp_path / "g.bin.2hop.bin")).csr()
+    assert rp.tolist() == [0, 3, 6, 9, 11, 13] and cols.tolist() == [0, 1, 2, 0, 1, 2, 0, 1, 2, 3, 4, 3, 4]
+    assert subprocess.run([exe, "-f", str(tmp_path / "g.bin"), "-h", "-2"], capture_output=True, timeout=60).returncode == 2
 
 
 FENNEL = json.load(open(os.path.join(ROOT, "tests", "golden", "fennel_vectors.json")))

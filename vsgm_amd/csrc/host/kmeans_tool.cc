@@ -92,6 +92,36 @@ Graph ClosedNeighbourhoods(const Graph& g) {
   return out;
 }
 
+Graph TwoHopBalls(const Graph& g) {
+  Graph out;
+  out.vertex_count = g.vertex_count;
+  std::vector<std::vector<uint32_t>> rows(g.vertex_count);
+#pragma omp parallel
+  {
+    std::vector<uint8_t> seen(g.vertex_count, 0);
+#pragma omp for schedule(dynamic, 256)
+    for (uint64_t v = 0; v < g.vertex_count; ++v) {
+      auto& r = rows[v];
+      auto add = [&](uint32_t u) {
+        if (!seen[u]) seen[u] = 1, r.push_back(u);
+      };
+      add((uint32_t)v);
+      for (uint64_t i = g.row_ptrs[v]; i < g.row_ptrs[v + 1]; ++i) {
+        const uint32_t w = g.cols[i];
+        add(w);
+        for (uint64_t j = g.row_ptrs[w]; j < g.row_ptrs[w + 1]; ++j) add(g.cols[j]);
+      }
+      for (uint32_t u : r) seen[u] = 0;
+      std::sort(r.begin(), r.end());
+    }
+  }
+  out.row_ptrs.assign(g.vertex_count + 1, 0);
+  for (uint64_t v = 0; v < g.vertex_count; ++v) out.row_ptrs[v + 1] = out.row_ptrs[v] + rows[v].size();
+  out.cols.reserve(out.row_ptrs.back());
+  for (auto& r : rows) out.cols.insert(out.cols.end(), r.begin(), r.end());
+  return out;
+}
+
 bool OpenIfClosed(Graph& g) {
   for (uint64_t v = 0; v < g.vertex_count; ++v)
     if (!std::binary_search(g.cols.begin() + g.row_ptrs[v], g.cols.begin() + g.row_ptrs[v + 1], (uint32_t)v)) return false;
@@ -293,14 +323,16 @@ int FeaturesToolMain(int argc, char** argv) {
   const std::string filename = str("-f", "./data/com-dblp.ungraph.txt");
   const int hop = (int)num("-h", 2);
   const int duplicates = (int)num("-dn", 0);
-  if (hop != 1 || duplicates != 0) {
-    std::cerr << "only 1-hop features (-h 1, no -dn) are written: the GPU k-means reads the closed rows they hold" << std::endl;
+  if ((hop != 1 && hop != 2) || duplicates != 0) {
+    std::cerr << "only -h 1 and -h 2 features are written (no sampled -h -2, no -dn duplication)" << std::endl;
     return 2;
   }
   Graph g = LoadGraph(filename, -1, 0, false);
-  // "<file>.1hop.bin" (features.cc:176-207): each row plus its own vertex, sorted
-  WriteCSRBinFile(filename + ".1hop.bin", ClosedNeighbourhoods(g));
-  std::cout << "wrote " << filename << ".1hop.bin" << std::endl;
+  // "<file>.1hop.bin" (features.cc:176-207): each row plus its own vertex;
+  // "<file>.2hop.bin" (:207-258): every vertex within two steps; sorted rows
+  const std::string out = filename + (hop == 1 ? ".1hop.bin" : ".2hop.bin");
+  WriteCSRBinFile(out, hop == 1 ? ClosedNeighbourhoods(g) : TwoHopBalls(g));
+  std::cout << "wrote " << out << std::endl;
   return 0;
 }
 

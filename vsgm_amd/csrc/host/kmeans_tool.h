@@ -19,6 +19,9 @@ void ReorderClusters(const Graph& g, int k, std::vector<int>& assignment);
 // Each row plus its own vertex, sorted: the ".1hop" graph of the features
 // tool (features.cc:176-207).
 Graph ClosedNeighbourhoods(const Graph& g);
+// Every vertex within two steps of v, v included, sorted: the ".2hop"
+// graph of the features tool (features.cc:207-258).
+Graph TwoHopBalls(const Graph& g);
 // When every row holds its own vertex, remove it once from each row (the
 // inverse of ClosedNeighbourhoods) and return true; otherwise leave g alone.
 bool OpenIfClosed(Graph& g);
