@@ -210,6 +210,10 @@ int vsgm_kmeans_tool(int argc, char** argv);
  * duplication are rejected. The kmeans tool reads such a file as the reference's
  * does (rows that hold their own vertex are clustered as they are). */
 int vsgm_features_tool(int argc, char** argv);
+/* The query_test tool (src/tools/query_test.cc:12-65): -p preset, -kc k or
+ * -km k; prints each query and its optimised plan (and, for graphlets, the
+ * hop of every plan and the largest). */
+int vsgm_query_test(int argc, char** argv);
 /* The fennel tool (src/tools/fennel.cc:21-101): -f graph -p parts -h 1|2;
  * writes <graph>.fennel.part.<p>, one part per vertex, the same parts as
  * the reference's streaming partitioner. vsgm_graph_fennel is the same

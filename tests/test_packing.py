@@ -195,3 +195,20 @@ def test_fennel_live_against_reference_tool(vsgm, tmp_path, seed):
         assert r.returncode == 0, r.stderr
         outs.append((d / f"g.bin.fennel.part.{parts}").read_bytes())
     assert outs[0] == outs[1]
+
+
+@pytest.mark.parametrize("flags", [["-p", "2"], ["-kc", "5"], ["-km", "4"]])
+def test_query_test_tool_prints_the_planner_plans(vsgm, flags):
+    out = subprocess.run([os.path.join(ROOT, "vsgm_amd", "bin", "query_test")] + flags, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    if flags[0] == "-p":
+        queries = [vsgm.Query.preset(int(flags[1]))]
+    elif flags[0] == "-kc":
+        queries = [vsgm.Query.clique(int(flags[1]))]
+    else:
+        queries = vsgm.Query.graphlets(int(flags[1]))
+    plans = [vsgm.Plan.optimize(q) for q in queries]
+    for p in plans:
+        assert p.serialize() in out.stdout
+    if flags[0] == "-km":
+        assert f"largest hop: {max(p.hop for p in plans)}" in out.stdout

@@ -15,6 +15,8 @@
 #include <string>
 
 #include "ingest.h"
+#include "plan.h"
+#include "query.h"
 #include "vsgm_hip.h"
 #include "vsgm_partition.h"
 
@@ -419,6 +421,51 @@ extern "C" int vsgm_fennel_tool(int argc, char** argv) {
     return vsgm::FennelToolMain(argc, argv);
   } catch (const std::exception& ex) {
     std::cerr << "fennel: " << ex.what() << std::endl;
+    return 1;
+  }
+}
+
+namespace vsgm {
+
+int QueryTestMain(int argc, char** argv) {
+  if (argc == 1) return -1;
+  auto num = [&](const char* key, long d) { return Flag(argc, argv, key) ? atol(Flag(argc, argv, key)) : d; };
+  const int preset = (int)num("-p", 1);
+  const int k_c = (int)num("-kc", 0);
+  const int k_m = (int)num("-km", 0);
+  std::vector<Query> queries;
+  if (k_c > 0)
+    queries.push_back(Query::Clique(k_c));
+  else if (k_m > 0)
+    queries = Query::Graphlets(k_m);
+  else
+    queries.push_back(Query::Preset(preset));
+  int largest = 0;
+  std::vector<int> hops;
+  for (auto& q : queries) {
+    std::cout << q.Describe();
+    Plan p(q);
+    p.Optimize();
+    std::cout << p.Serialize();
+    hops.push_back(p.GetHop());
+    largest = std::max(largest, p.GetHop());
+  }
+  // the graphlet summary of query_test.cc:56-60
+  if (k_c == 0 && k_m > 0) {
+    std::cout << "-----------" << std::endl;
+    for (size_t i = 0; i < hops.size(); ++i) std::cout << i << ": " << hops[i] << std::endl;
+    std::cout << "largest hop: " << largest << std::endl;
+  }
+  return 0;
+}
+
+}  // namespace vsgm
+
+extern "C" int vsgm_query_test(int argc, char** argv) {
+  try {
+    return vsgm::QueryTestMain(argc, argv);
+  } catch (const std::exception& ex) {
+    std::cerr << "query_test: " << ex.what() << std::endl;
     return 1;
   }
 }
