@@ -14,7 +14,7 @@
 
 #include "graph.h"
 #include "ingest.h"
-#include "kmeans_tool.h"
+#include "tools.h"
 #include "packing.h"
 #include "plan.h"
 #include "query.h"

@@ -8,7 +8,7 @@
 #include "executor.h"
 #include "graph.h"
 #include "ingest.h"
-#include "kmeans_tool.h"
+#include "tools.h"
 #include "packing.h"
 #include "plan.h"
 #include "query.h"

@@ -1,7 +1,9 @@
-// The kmeans tool (src/tools/kmeans.cc:636-694) on the GPU: reads a .bin
-// graph, clusters its vertices, writes one cluster id per line to
-// `<graph>.kmeans.<k>` (KMeans::Dump, kmeans.cc:559-572).
-#include "kmeans_tool.h"
+// The reference's offline partitioning tools over this library
+// (src/tools/*.cc): kmeans on the GPU (one id per line to
+// `<graph>.kmeans.<k>`, KMeans::Dump, kmeans.cc:559-572, one or more
+// levels), features (`.1hop.bin` / `.2hop.bin`), fennel
+// (`.fennel.part.<p>`) and query_test (plans).
+#include "tools.h"
 
 #include <algorithm>
 #include <cfloat>

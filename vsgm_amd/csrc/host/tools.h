@@ -1,5 +1,6 @@
-// Vertex clustering for view bins on the GPU and the kmeans tool
-// (src/tools/kmeans.cc:636-694) over it.
+// Vertex partitioning for view bins — GPU k-means (one or more levels),
+// Fennel, the feature graphs — behind the reference's kmeans, features,
+// fennel and query_test tools (src/tools/*.cc).
 #pragma once
 
 #include <cstdint>
