@@ -180,7 +180,7 @@ def test_large_hub_graph(vsgm, oracle, gpu):
 
 
 def test_full_size_triangles_and_cliques(vsgm, oracle, gpu):
-    """BASELINE sizes: R-MAT scale 22 triangles against the textbook counter
+    """Full-size inputs: R-MAT scale 22 triangles against the textbook counter
     on the host cores, scale 20 diamonds and scale 18 4-cliques likewise,
     plus count invariance under a forced multi-batch run."""
     g = vsgm.Graph.rmat(22, 16, seed=1)

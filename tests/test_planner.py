@@ -131,7 +131,7 @@ def test_from_order_rejects_disconnected_order(vsgm):
         vsgm.Plan.from_order(q, bad)
 
 
-def test_hop_and_root_degree_of_baseline_patterns(vsgm):
+def test_hop_and_root_degree_of_benchmark_patterns(vsgm):
     want = {"clique 3": (2, 2), "clique 4": (2, 3), "preset 2": (2, 3), "preset 1": (2, 2), "preset 4": (2, 3)}
     for spec, (hop, rd) in want.items():
         p = vsgm.Plan.optimize(_query_for(vsgm, spec))
