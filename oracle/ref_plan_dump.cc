@@ -9,7 +9,7 @@
 //   graphlets <k>     every query of Graphlet(k)             (pattern_matching_pipeline.cu:65)
 //   adj <n> <bits>    Query(n, m, adj) with an n*n 0/1 matrix in row-major order
 // and writes, to the file named by argv[1], one record per query in the
-// key/value format of the build's Plan::Serialize, after "query: <spec>".
+// key/value format of vsgm_amd's Plan::Serialize, after "query: <spec>".
 // <algorithm> first: the reference's partial_order.h relies on it being
 // included before it (query.h does so).
 #include <algorithm>

@@ -1,4 +1,4 @@
-"""Planner parity: the build's planner against the REFERENCE planner.
+"""Planner parity: vsgm_amd's planner against the REFERENCE planner.
 
 tests/golden/ref_plans.txt holds the reference's own plans (remapped
 adjacency, conditions, hop, root degree, exec sequence, per-level vertex

@@ -64,7 +64,7 @@ int Run(int argc, char** argv) {
   const int do_reorder = fl.i("-dr", 0);
   const int do_split = fl.i("-ds", 0);
   const int split_times = fl.i("-dst", 1);
-  // Options of this build: -gk K partitions the vertices with the GPU k-means
+  // Options beyond the reference's: -gk K partitions the vertices with the GPU k-means
   // (K clusters, -gki iterations, -gks seed) instead of reading the .vbmap
   // file; -dv 1 builds each bin's view on the consumer's GPU from the whole
   // graph kept resident there, instead of materialising it on the host.
