@@ -158,6 +158,10 @@ int vsgm_executor_add_view(vsgm_executor* e, const vsgm_view_holder* h, int* ind
 int vsgm_executor_set_graph(vsgm_executor* e, const vsgm_graph* g);
 int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree,
                                   int* index);
+/* The device-built counterpart of vsgm_executor_transfer: builds the bin's
+ * view from the set_graph graph and replaces every resident view with it
+ * (what pm_pipeline -dv 1 does per bin). */
+int vsgm_executor_transfer_device(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree);
 int vsgm_executor_view_csr(const vsgm_executor* e, int view, uint64_t* row_ptrs, uint32_t* cols, uint64_t cols_cap, uint64_t* entries);
 int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans);
 int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint64_t* n_nodes);

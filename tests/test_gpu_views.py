@@ -44,6 +44,12 @@ def test_device_views_count_like_host_views(vsgm, oracle, gpu):
         ex.add_device_view(np.nonzero(assign == b)[0].astype(np.uint32), hop, rd)
     ex.match_all()
     assert ex.total_counts == want
+    # one bin at a time, each replacing the last (the -dv 1 consumer loop)
+    ex.reset()
+    for b in range(6):
+        ex.transfer_device(np.nonzero(assign == b)[0].astype(np.uint32), hop, rd)
+        ex.match()
+    assert ex.total_counts == want
     # an empty bin builds an empty view and counts nothing
     ex.reset()
     i = ex.add_device_view(np.zeros(0, np.uint32), hop, rd)

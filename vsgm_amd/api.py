@@ -134,6 +134,7 @@ def _declare(L):
         "vsgm_executor_add_view": ([P, P, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_set_graph": ([P, P], C.c_int),
         "vsgm_executor_add_device_view": ([P, P, u64, C.c_int, u64, C.POINTER(C.c_int)], C.c_int),
+        "vsgm_executor_transfer_device": ([P, P, u64, C.c_int, u64], C.c_int),
         "vsgm_executor_view_csr": ([P, C.c_int, P, P, u64, C.POINTER(u64)], C.c_int),
         "vsgm_executor_match_all": ([P, P, C.c_int], C.c_int),
         "vsgm_executor_capture": ([P, P, C.c_int, C.POINTER(u64)], C.c_int),
@@ -559,6 +560,11 @@ class PipelineExecutor(_Handle):
         i = C.c_int()
         _check(lib().vsgm_executor_add_device_view(self._h, _ptr(s), len(s), hop, root_degree, C.byref(i)), "add_device_view")
         return i.value
+
+    def transfer_device(self, sources, hop: int, root_degree: int):
+        """Build one bin's view on the device and make it the only resident view (like transfer)."""
+        s = np.ascontiguousarray(np.asarray(sources, dtype=np.uint32))
+        _check(lib().vsgm_executor_transfer_device(self._h, _ptr(s), len(s), hop, root_degree), "transfer_device")
 
     def view_csr(self, view: int, vertex_count: int):
         n = u64()

@@ -428,6 +428,13 @@ int vsgm_executor_set_graph(vsgm_executor* e, const vsgm_graph* g) {
   return Guard([&] { e->e->SetGraph(g->g); });
 }
 
+int vsgm_executor_transfer_device(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree) {
+  return Guard([&] {
+    VSGM_CHECK(e && (sources || n_sources == 0), "bad arguments");
+    e->e->TransferDevice(std::vector<vsgm::uintV>(sources, sources + n_sources), hop, root_degree);
+  });
+}
+
 int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree,
                                   int* index) {
   return Guard([&] {
