@@ -191,3 +191,32 @@ def test_preprocess_graph_text_outputs(vsgm, tmp_path):
     assert (tmp_path / "g.txt.adj.txt").read_text() == "0\t1 2 \n1\t0 2 \n2\t0 1 \n3\t4 \n4\t3 \n"
     assert (tmp_path / "g.txt.adj.txt.info").read_text() == "5 8"
     assert subprocess.run([exe, "-f", str(path), "-o", "3"], capture_output=True, timeout=60).returncode == 2
+
+
+from hypothesis import given, settings, HealthCheck, strategies as st  # noqa: E402
+
+_line = st.one_of(
+    st.tuples(st.integers(0, 60), st.integers(0, 60), st.sampled_from(["\t", " ", "  ", " \t"]), st.sampled_from(["", " 1.5", "\t7"]),
+              st.sampled_from(["\n", "\r\n"])).map(lambda t: f"{t[0] + 1000}{t[2]}{t[1] + 1000}{t[3]}{t[4]}"),
+    st.sampled_from(["# comment\n", "% header\n", "\n", "c not an edge\n"]),
+)
+
+
+@settings(max_examples=60, deadline=None, suppress_health_check=[HealthCheck.function_scoped_fixture])
+@given(lines=st.lists(_line, min_size=1, max_size=80), threads=st.integers(1, 9))
+def test_snap_reader_matches_oracle_on_random_texts(vsgm, tmp_path, lines, threads):
+    text = "".join(lines)
+    path = tmp_path / "g.txt"
+    path.write_text(text)
+    try:
+        pairs, V = I.parse_snap(text)
+    except ValueError:
+        with pytest.raises(vsgm.VsgmError):
+            vsgm.Graph.read_snap(str(path), threads=threads)
+        return
+    rp, cols = vsgm.Graph.load(str(path), threads=threads).csr()
+    want = I.csr_from_pairs(pairs, V, True, False)
+    assert np.array_equal(rp, want[0]) and np.array_equal(cols, want[1])
+    rp, cols = vsgm.Graph.read_snap(str(path), threads=threads).csr()
+    want = I.csr_from_pairs(pairs, V, True, True)
+    assert np.array_equal(rp, want[0]) and np.array_equal(cols, want[1])

@@ -7,6 +7,7 @@ library call, error paths and invariants."""
 import json
 import os
 import subprocess
+import tempfile
 
 import numpy as np
 import pytest
@@ -212,3 +213,36 @@ def test_query_test_tool_prints_the_planner_plans(vsgm, flags):
         assert p.serialize() in out.stdout
     if flags[0] == "-km":
         assert f"largest hop: {max(p.hop for p in plans)}" in out.stdout
+
+
+from hypothesis import given, settings, HealthCheck, strategies as st  # noqa: E402
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_TOOL) and os.path.exists(REF_FENNEL)),
+                    reason="the reference tools are built only where the reference tree exists")
+@settings(max_examples=25, deadline=None, suppress_health_check=[HealthCheck.function_scoped_fixture])
+@given(edges=st.lists(st.tuples(st.integers(0, 40), st.integers(0, 40)), min_size=1, max_size=150),
+       first=st.integers(1, 4), hop=st.integers(1, 3), pool=st.integers(1, 3), order=st.integers(0, 1), slack=st.floats(1.0, 3.0),
+       parts=st.integers(1, 6), fhop=st.integers(1, 2), seed=st.integers(0, 1000))
+def test_random_graphs_pack_and_partition_like_the_reference_tools(vsgm, tmp_path, edges, first, hop, pool, order, slack, parts, fhop,
+                                                                   seed):
+    g = vsgm.Graph.from_edges(41, np.array(edges, np.uint32))
+    rp, cl = g.csr()
+    # a limit every single source fits: its ball is at most the whole graph
+    limit = max(int(len(cl) * slack), 1)
+    part = np.random.default_rng(seed).integers(0, first, g.vertex_count)
+    outs = []
+    for name, pack_tool, fennel_tool in (("ref", REF_TOOL, REF_FENNEL),
+                                         ("me", TOOL, os.path.join(ROOT, "vsgm_amd", "bin", "fennel"))):
+        # a fresh directory per example: each tool keeps a degree cache beside the graph
+        d = tempfile.mkdtemp(prefix=name, dir=str(tmp_path))
+        gf, pf = os.path.join(d, "g.bin"), os.path.join(d, f"g.bin.kmeans.{first}")
+        g.write_bin(gf)
+        np.savetxt(pf, part, fmt="%d")
+        r = run_tool(pack_tool, gf, pf, first, hop, limit * 4 / 1e9, pool, order)
+        f = subprocess.run([fennel_tool, "-f", gf, "-p", str(parts), "-h", str(fhop)], capture_output=True, timeout=60)
+        base = f"{gf}.{hop}.hop"
+        outs.append((r.returncode, open(base + ".vbmap").read() if r.returncode == 0 else None,
+                     open(base + ".vbstat").read() if r.returncode == 0 else None, f.returncode,
+                     open(f"{gf}.fennel.part.{parts}").read()))
+    assert outs[0] == outs[1]
