@@ -210,8 +210,9 @@ int vsgm_view_packing(int argc, char** argv);
 int vsgm_kmeans_tool(int argc, char** argv);
 /* The features tool (src/tools/features.cc:68-258): -f graph -h 1 writes
  * <graph>.1hop.bin (every row plus its own vertex), -h 2 <graph>.2hop.bin
- * (every vertex within two steps), rows sorted. The sampled -h -2 and -dn
- * duplication are rejected. The kmeans tool reads such a file as the reference's
+ * (every vertex within two steps), rows sorted; -dn n writes
+ * <graph>.x<n>.bin, n copies with shuffled ids. The sampled -h -2 is
+ * rejected. The kmeans tool reads such a file as the reference's
  * does (rows that hold their own vertex are clustered as they are). */
 int vsgm_features_tool(int argc, char** argv);
 /* The query_test tool (src/tools/query_test.cc:12-65): -p preset, -kc k or

@@ -137,17 +137,19 @@ REF_FEATURES = os.path.join(ROOT, "oracle", "_ref", "features")
 
 
 @pytest.mark.skipif(not os.path.exists(REF_FEATURES), reason="oracle/_ref/features is built only where the reference tree exists")
-@pytest.mark.parametrize("scale,ef,seed,hop", [(9, 8, 1, 1), (11, 4, 2, 1), (9, 8, 3, 2), (10, 4, 4, 2)])
-def test_features_tool_matches_reference_tool(vsgm, tmp_path, scale, ef, seed, hop):
+@pytest.mark.parametrize("scale,ef,seed,flags,out", [(9, 8, 1, ["-h", "1"], "1hop"), (11, 4, 2, ["-h", "1"], "1hop"),
+                                                     (9, 8, 3, ["-h", "2"], "2hop"), (10, 4, 4, ["-h", "2"], "2hop"),
+                                                     (8, 8, 5, ["-dn", "3"], "x3"), (10, 4, 6, ["-dn", "2"], "x2")])
+def test_features_tool_matches_reference_tool(vsgm, tmp_path, scale, ef, seed, flags, out):
     g = vsgm.Graph.rmat(scale, ef, seed=seed)
     outs = []
     for name, tool in (("ref", REF_FEATURES), ("me", os.path.join(ROOT, "vsgm_amd", "bin", "features"))):
         d = tmp_path / name
         d.mkdir()
         g.write_bin(str(d / "g.bin"))
-        r = subprocess.run([tool, "-f", str(d / "g.bin"), "-h", str(hop), "-t", "4"], capture_output=True, text=True, timeout=120)
+        r = subprocess.run([tool, "-f", str(d / "g.bin"), "-t", "4"] + flags, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
-        outs.append((d / f"g.bin.{hop}hop.bin").read_bytes())
+        outs.append((d / f"g.bin.{out}.bin").read_bytes())
     assert outs[0] == outs[1]
 
 

@@ -14,6 +14,7 @@
 #include <fstream>
 #include <iostream>
 #include <numeric>
+#include <random>
 #include <string>
 
 #include "ingest.h"
@@ -123,6 +124,44 @@ Graph TwoHopBalls(const Graph& g) {
   for (uint64_t v = 0; v < g.vertex_count; ++v) out.row_ptrs[v + 1] = out.row_ptrs[v] + rows[v].size();
   out.cols.reserve(out.row_ptrs.back());
   for (auto& r : rows) out.cols.insert(out.cols.end(), r.begin(), r.end());
+  return out;
+}
+
+Graph Duplicate(const Graph& g, int copies) {
+  VSGM_CHECK(copies >= 1 && (uint64_t)copies * g.vertex_count <= 0xFFFFFFFFull, "bad duplicate count");
+  const uint64_t V = g.vertex_count, N = V * (uint64_t)copies;
+  // copy d of vertex v is v + V * d, its list shifted alike (features.cc:89-125)
+  auto old_row = [&](uint64_t u, std::vector<uint32_t>& out) {
+    const uint64_t v = u % V, d = u / V;
+    for (uint64_t i = g.row_ptrs[v]; i < g.row_ptrs[v + 1]; ++i) out.push_back((uint32_t)(g.cols[i] + V * d));
+  };
+  // ids shuffled with the standard library's default engine, default seed
+  // (features.cc:131-137), so the file matches the reference tool's
+  std::vector<uint32_t> new2old(N);
+  std::iota(new2old.begin(), new2old.end(), 0u);
+  std::shuffle(new2old.begin(), new2old.end(), std::default_random_engine{});
+  std::vector<uint32_t> old2new(N);
+  for (uint64_t u = 0; u < N; ++u) old2new[new2old[u]] = (uint32_t)u;
+  Graph out;
+  out.vertex_count = N;
+  out.row_ptrs.assign(N + 1, 0);
+  for (uint64_t u = 0; u < N; ++u) {
+    const uint64_t v = new2old[u] % V;
+    out.row_ptrs[u + 1] = out.row_ptrs[u] + (g.row_ptrs[v + 1] - g.row_ptrs[v]);
+  }
+  out.cols.resize(out.row_ptrs[N]);
+#pragma omp parallel
+  {
+    std::vector<uint32_t> row;
+#pragma omp for schedule(dynamic, 1024)
+    for (uint64_t u = 0; u < N; ++u) {
+      row.clear();
+      old_row(new2old[u], row);
+      for (auto& c : row) c = old2new[c];
+      std::sort(row.begin(), row.end());
+      std::copy(row.begin(), row.end(), out.cols.begin() + out.row_ptrs[u]);
+    }
+  }
   return out;
 }
 
@@ -327,11 +366,18 @@ int FeaturesToolMain(int argc, char** argv) {
   const std::string filename = str("-f", "./data/com-dblp.ungraph.txt");
   const int hop = (int)num("-h", 2);
   const int duplicates = (int)num("-dn", 0);
-  if ((hop != 1 && hop != 2) || duplicates != 0) {
-    std::cerr << "only -h 1 and -h 2 features are written (no sampled -h -2, no -dn duplication)" << std::endl;
+  Graph g = LoadGraph(filename, -1, 0, false);
+  if (duplicates > 0) {
+    // "<file>.x<dn>.bin": dn shuffled copies of the graph, -h ignored (features.cc:89-173)
+    const std::string out = filename + ".x" + std::to_string(duplicates) + ".bin";
+    WriteCSRBinFile(out, Duplicate(g, duplicates));
+    std::cout << "wrote " << out << std::endl;
+    return 0;
+  }
+  if (hop != 1 && hop != 2) {
+    std::cerr << "only -h 1 and -h 2 features are written (not the sampled -h -2)" << std::endl;
     return 2;
   }
-  Graph g = LoadGraph(filename, -1, 0, false);
   // "<file>.1hop.bin" (features.cc:176-207): each row plus its own vertex;
   // "<file>.2hop.bin" (:207-258): every vertex within two steps; sorted rows
   const std::string out = filename + (hop == 1 ? ".1hop.bin" : ".2hop.bin");

@@ -23,6 +23,9 @@ Graph ClosedNeighbourhoods(const Graph& g);
 // Every vertex within two steps of v, v included, sorted: the ".2hop"
 // graph of the features tool (features.cc:207-258).
 Graph TwoHopBalls(const Graph& g);
+// `copies` disjoint copies of g with the vertex ids shuffled as the
+// reference's features -dn does (features.cc:89-173).
+Graph Duplicate(const Graph& g, int copies);
 // When every row holds its own vertex, remove it once from each row (the
 // inverse of ClosedNeighbourhoods) and return true; otherwise leave g alone.
 bool OpenIfClosed(Graph& g);
