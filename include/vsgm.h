@@ -206,7 +206,7 @@ int vsgm_view_packing(int argc, char** argv);
  * -rc reorder -init 0, plus -g GPU and -s seed (the reference seeds with
  * the clock); writes one cluster id per line to <prefix><K>, empty clusters
  * removed. Multi-level "axb" and "a^x" levels are clustered level by level;
- * "2^x" (the reference's recursive bisection) is rejected. */
+ * "2^x" is the reference's recursive bisection (Run2x, kmeans.cc:284-383). */
 int vsgm_kmeans_tool(int argc, char** argv);
 /* The features tool (src/tools/features.cc:68-258): -f graph -h 1 writes
  * <graph>.1hop.bin (every row plus its own vertex), -h 2 <graph>.2hop.bin
@@ -229,8 +229,9 @@ int vsgm_graph_fennel(const vsgm_graph* g, int parts, int hop, int32_t* assignme
  * IO): levels "K", "AxBx..." or "A^X" (A != 2), KMeans::Run(std::string),
  * kmeans.cc:253-281 and Run(std::vector<int>), :386-489 — every level
  * splits each non-empty cluster c of the previous one into K with
- * vsgm_kmeans_members, ids a + c * K. Writes int32[V] ids and the id range
- * (*clusters; empty clusters are not removed). */
+ * vsgm_kmeans_members, ids a + c * K — or "2^X", the recursive bisection
+ * of Run2x (:284-383). Writes int32[V] ids and the id range (*clusters;
+ * empty clusters are not removed). */
 int vsgm_graph_kmeans(const vsgm_graph* g, const char* levels, int iters, uint64_t seed, int device, int reorder, int32_t* assignment,
                       int* clusters);
 /* The tool's two host steps on an assignment (int32[V], ids < k):

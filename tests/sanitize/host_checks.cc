@@ -135,6 +135,8 @@ void ViewsAndPacking() {
          "cluster level strings");
   for (const char* bad : {"", "2^3", "4x", "x4", "17", "4y2", "99999999"})
     ExpectThrow([&] { ParseLevels(bad); }, "bad cluster level string rejected");
+  Expect(ParseBisection("2^3") == 3 && ParseBisection("4x2") == 0 && ParseBisection("3^2") == 0, "bisection strings");
+  for (const char* bad : {"2^0", "2^17", "2^x", "2^999"}) ExpectThrow([&] { ParseBisection(bad); }, "bad bisection string rejected");
   Graph closed = ClosedNeighbourhoods(g);
   Expect(closed.cols.size() == g.cols.size() + g.vertex_count, "closed rows add one entry per vertex");
   Graph opened = closed;

@@ -143,8 +143,9 @@ def test_member_run_matches_checker(vsgm, small):
 
 def test_multi_level_nests_and_keeps_cliques_whole(vsgm, gpu):
     """"3x2" on six disjoint 5-cliques: level 2 splits level 1's clusters
-    (id // 2 is the level-1 cluster), no clique is ever split, "3" equals
-    the single-level run, and "2^2" (recursive bisection) is refused."""
+    (id // 2 is the level-1 cluster), no clique is ever split, and "3"
+    equals the single-level run. "2^3" (recursive bisection) on eight
+    cliques keeps every clique whole and numbers its clusters 0..n-1."""
     from vsgm_amd.partition import kmeans
 
     m, n = 5, 6
@@ -154,5 +155,10 @@ def test_multi_level_nests_and_keeps_cliques_whole(vsgm, gpu):
     two, k2 = vsgm.kmeans_levels(g, "3x2", iters=3, seed=4)
     assert k2 == 6 and np.array_equal(two // 2, one)
     assert all(len(set(two[c * m:(c + 1) * m].tolist())) == 1 for c in range(n))
+    m, n = 4, 8
+    g8 = vsgm.Graph.from_edges(m * n, [(c * m + i, c * m + j) for c in range(n) for i in range(m) for j in range(i + 1, m)])
+    b, nb = vsgm.kmeans_levels(g8, "2^3", iters=3, seed=2)
+    assert 2 <= nb <= 8 and sorted(set(b.tolist())) == list(range(nb))
+    assert all(len(set(b[c * m:(c + 1) * m].tolist())) == 1 for c in range(n))
     with pytest.raises(vsgm.VsgmError):
-        vsgm.kmeans_levels(g, "2^2")
+        vsgm.kmeans_levels(g, "2^0")

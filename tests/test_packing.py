@@ -117,7 +117,7 @@ def test_cluster_reorder_and_compaction(vsgm):
 
 def test_kmeans_tool_rejects_bad_levels_before_any_work(tmp_path):
     exe = os.path.join(ROOT, "vsgm_amd", "bin", "kmeans")
-    for levels in ("2^3", "4x", "0x", "17"):
+    for levels in ("2^0", "2^17", "4x", "0x", "17"):
         out = subprocess.run([exe, "-f", str(tmp_path / "missing.bin"), "-k", levels], capture_output=True, text=True, timeout=60)
         assert out.returncode == 2, (levels, out.stderr)
 

@@ -253,7 +253,7 @@ int vsgm_graph_kmeans(const vsgm_graph* g, const char* levels, int iters, uint64
                       int* clusters) {
   return Guard([&] {
     VSGM_CHECK(g && levels && assignment && clusters && iters >= 0, "bad arguments");
-    std::vector<int> a = vsgm::GpuKMeansLevels(g->g, vsgm::ParseLevels(levels), iters, seed, device, reorder != 0, clusters);
+    std::vector<int> a = vsgm::GpuKMeansSpec(g->g, levels, iters, seed, device, reorder != 0, clusters);
     std::copy(a.begin(), a.end(), assignment);
   });
 }

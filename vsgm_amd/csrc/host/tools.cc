@@ -186,6 +186,15 @@ bool OpenIfClosed(Graph& g) {
   return true;
 }
 
+int ParseBisection(const std::string& ks) {
+  if (ks.size() < 3 || ks.compare(0, 2, "2^") != 0) return 0;
+  const std::string x = ks.substr(2);
+  VSGM_CHECK(!x.empty() && x.size() < 3 && x.find_first_not_of("0123456789") == std::string::npos, "bad level count in \"" + ks + "\"");
+  const int n = std::stoi(x);
+  VSGM_CHECK(n >= 1 && n <= 16, "bad level count in \"" + ks + "\"");
+  return n;
+}
+
 std::vector<int> ParseLevels(const std::string& ks) {
   auto number = [&](const std::string& t) {
     VSGM_CHECK(!t.empty() && t.size() < 6 && t.find_first_not_of("0123456789") == std::string::npos, "bad cluster count in \"" + ks + "\"");
@@ -195,9 +204,8 @@ std::vector<int> ParseLevels(const std::string& ks) {
   const size_t caret = ks.find('^');
   if (caret != std::string::npos) {
     const int a = number(ks.substr(0, caret)), x = number(ks.substr(caret + 1));
-    // "2^x" is the reference's recursive bisection with centroid-ordered
-    // halves (Run2x, kmeans.cc:284-383), a different algorithm
-    VSGM_CHECK(a != 2, "recursive bisection (\"2^x\") is not supported; use \"2x2x...\"");
+    // "2^x" is the recursive bisection of GpuKMeansBisect
+    VSGM_CHECK(a != 2, "\"2^x\" is a recursive bisection, not a list of levels");
     VSGM_CHECK(x >= 1 && x <= 8, "bad level count in \"" + ks + "\"");
     levels.assign(x, a);
   } else {
@@ -215,6 +223,145 @@ std::vector<int> ParseLevels(const std::string& ks) {
   }
   VSGM_CHECK(total <= (1u << 30), "too many clusters");
   return levels;
+}
+
+namespace {
+
+// Recursive bisection (KMeans::Run2x, kmeans.cc:284-383). Each node splits
+// its vertices in two with the member-subset k-means, groups them cluster
+// 0 first (the reference's in-place two-pointer swap), recurses on both
+// halves, and then orients the halves so the two most similar child-level
+// centroids meet at the boundary. A centroid is kept as its member list;
+// the similarity of two is the dot product of their mean features
+// (CosineSimilarity's dense-dense similarity, kmeans_utils.h:152-158).
+class Bisector {
+ public:
+  Bisector(const Graph& g, vsgm_ctx* ctx, const void* rp, const void* cols, void* dm, int iters, uint64_t seed, int levels,
+           std::vector<int>& global)
+      : g_(g), ctx_(ctx), rp_(rp), cols_(cols), dm_(dm), iters_(iters), seed_(seed), levels_(levels), global_(global),
+        acc_(g.vertex_count, 0.0) {}
+
+  int Run(uint32_t* vids, uint64_t n, int level, std::vector<uint32_t> (&cent)[2]) {
+    std::vector<int32_t> a(n, 0);
+    uint64_t size0 = n;
+    if (n >= 2) {
+      std::vector<uint64_t> sz(2, 0);
+      const uint64_t s = seed_ + 0x9E3779B97F4A7C15ull * (++nodes_);
+      Check(vsgm_h2d(ctx_, dm_, vids, n * sizeof(uint32_t)), "member upload");
+      Check(vsgm_kmeans_members(ctx_, (const uint64_t*)rp_, (const uint32_t*)cols_, g_.vertex_count, (const uint32_t*)dm_, n, 2, iters_, s,
+                                a.data(), sz.data(), nullptr),
+            "k-means");
+      size0 = sz[0];
+    }
+    if (size0 == 0 || size0 == n) {  // cannot be split: one cluster, both centroids its own
+      cent[0].assign(vids, vids + n);
+      cent[1] = cent[0];
+      return 1;
+    }
+    for (uint64_t i = 0, j = n - 1; i < j;) {
+      while (i < j && a[i] == 0) ++i;
+      while (i < j && a[j] == 1) --j;
+      if (i < j) std::swap(vids[i++], vids[j--]);
+    }
+    const uint64_t size1 = n - size0;
+    if (level == levels_) {
+      for (uint64_t i = size0; i < n; ++i) global_[vids[i]] = 1;
+      cent[0].assign(vids, vids + size0);
+      cent[1].assign(vids + size0, vids + n);
+      return 2;
+    }
+    std::vector<uint32_t> c0[2], c1[2];
+    const int n0 = Run(vids, size0, level + 1, c0);
+    const int n1 = Run(vids + size0, size1, level + 1, c1);
+    const double d[4] = {Dot(c0[0], c1[0]), Dot(c0[0], c1[1]), Dot(c0[1], c1[0]), Dot(c0[1], c1[1])};
+    const int min_idx = (int)(std::min_element(d, d + 4) - d);
+    const int i0 = (min_idx & 2) != 0, i1 = (min_idx & 1) != 0;
+    cent[0] = std::move(c0[1 - i0]);
+    cent[1] = std::move(c1[1 - i1]);
+    if (i0 != 1) std::reverse(vids, vids + size0);
+    if (i1 != 0) std::reverse(vids + size0, vids + n);
+    for (uint64_t i = size0; i < n; ++i) global_[vids[i]] += n0;
+    return n0 + n1;
+  }
+
+ private:
+  void Check(int st, const char* what) {
+    if (st != VSGM_OK) throw Error(std::string(what) + " failed: " + vsgm_last_error(ctx_));
+  }
+  // mean feature of A . mean feature of B, features 1/sqrt(|N[v]|) on N[v] = row + v
+  double Dot(const std::vector<uint32_t>& A, const std::vector<uint32_t>& B) {
+    auto each = [&](uint32_t v, auto&& f) {
+      const double w = std::sqrt(1.0 / (double)(g_.row_ptrs[v + 1] - g_.row_ptrs[v] + 1));
+      for (uint64_t i = g_.row_ptrs[v]; i < g_.row_ptrs[v + 1]; ++i) f(g_.cols[i], w);
+      f(v, w);
+    };
+    for (uint32_t v : A) each(v, [&](uint32_t u, double w) { acc_[u] += w; });
+    double dot = 0.0;
+    for (uint32_t v : B) each(v, [&](uint32_t u, double w) { dot += acc_[u] * w; });
+    for (uint32_t v : A) each(v, [&](uint32_t u, double) { acc_[u] = 0.0; });
+    return dot / ((double)A.size() * (double)B.size());
+  }
+
+  const Graph& g_;
+  vsgm_ctx* ctx_;
+  const void *rp_, *cols_;
+  void* dm_;
+  int iters_;
+  uint64_t seed_;
+  int levels_;
+  std::vector<int>& global_;
+  std::vector<double> acc_;
+  uint64_t nodes_ = 0;
+};
+
+}  // namespace
+
+std::vector<int> GpuKMeansBisect(const Graph& g, int levels, int iters, uint64_t seed, int device, int* clusters) {
+  const uint64_t V = g.vertex_count;
+  VSGM_CHECK(V > 0 && levels >= 1, "bad bisection arguments");
+  vsgm_ctx* ctx = nullptr;
+  if (vsgm_ctx_create(device, 0, &ctx) != VSGM_OK) throw Error("GPU k-means needs GPU " + std::to_string(device));
+  std::vector<int> global(V, 0);
+  int n = 0;
+  try {
+    void *rp = nullptr, *cols = nullptr, *dm = nullptr;
+    auto check = [&](int st, const char* what) {
+      if (st != VSGM_OK) throw Error(std::string(what) + " failed: " + vsgm_last_error(ctx));
+    };
+    check(vsgm_alloc(ctx, (V + 1) * sizeof(uint64_t), &rp), "k-means allocation");
+    check(vsgm_alloc(ctx, std::max<size_t>(g.cols.size(), 1) * sizeof(uint32_t), &cols), "k-means allocation");
+    check(vsgm_alloc(ctx, V * sizeof(uint32_t), &dm), "k-means allocation");
+    check(vsgm_h2d(ctx, rp, g.row_ptrs.data(), (V + 1) * sizeof(uint64_t)), "graph upload");
+    check(vsgm_h2d(ctx, cols, g.cols.data(), g.cols.size() * sizeof(uint32_t)), "graph upload");
+    std::vector<uint32_t> all(V);
+    std::iota(all.begin(), all.end(), 0u);
+    std::vector<uint32_t> cent[2];
+    Bisector b(g, ctx, rp, cols, dm, iters, seed, levels, global);
+    n = b.Run(all.data(), V, 1, cent);
+    // ids renumbered by runs along the final vertex order (kmeans.cc:364-376)
+    int prev = global[all[0]];
+    global[all[0]] = 0;
+    for (uint64_t i = 1; i < V; ++i) {
+      int& a = global[all[i]];
+      if (prev != a) {
+        prev = a;
+        a = global[all[i - 1]] + 1;
+      } else {
+        a = global[all[i - 1]];
+      }
+    }
+  } catch (...) {
+    vsgm_ctx_destroy(ctx);
+    throw;
+  }
+  vsgm_ctx_destroy(ctx);  // frees the pool
+  if (clusters) *clusters = n;
+  return global;
+}
+
+std::vector<int> GpuKMeansSpec(const Graph& g, const std::string& ks, int iters, uint64_t seed, int device, bool reorder, int* clusters) {
+  if (const int x = ParseBisection(ks)) return GpuKMeansBisect(g, x, iters, seed, device, clusters);
+  return GpuKMeansLevels(g, ParseLevels(ks), iters, seed, device, reorder, clusters);
 }
 
 std::vector<int> GpuKMeansLevels(const Graph& g, const std::vector<int>& levels, int iters, uint64_t seed, int device, bool reorder,
@@ -316,9 +463,8 @@ int KMeansToolMain(int argc, char** argv) {
   const int init = (int)num("-init", 0);
   const int device = (int)num("-g", 0);
   const uint64_t seed = (uint64_t)num("-s", 1);
-  std::vector<int> levels;
   try {
-    levels = ParseLevels(ks);
+    if (!ParseBisection(ks)) ParseLevels(ks);
   } catch (const std::exception& ex) {
     std::cerr << ex.what() << std::endl;
     return 2;
@@ -334,7 +480,7 @@ int KMeansToolMain(int argc, char** argv) {
   if (OpenIfClosed(g)) std::cout << "kmeans: rows hold their own vertex (a features -h 1 graph); clustering their closed rows" << std::endl;
   auto t0 = std::chrono::steady_clock::now();
   int k = 0;
-  std::vector<int> assignment = GpuKMeansLevels(g, levels, iters, seed, device, reorder != 0, &k);
+  std::vector<int> assignment = GpuKMeansSpec(g, ks, iters, seed, device, reorder != 0, &k);
   const int kept = RemoveEmptyClusters(k, assignment);
   std::cout << "kmeans: " << kept << " non-empty clusters of " << k << " in "
             << std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() << " ms" << std::endl;

@@ -44,6 +44,15 @@ std::vector<int> GpuKMeansLevels(const Graph& g, const std::vector<int>& levels,
 // above 1.1 V / parts skipped; the reference's float and double arithmetic
 // is kept so the same graph gives the same parts.
 std::vector<int> FennelPartition(const Graph& g, int parts, int hop);
+// x of "2^x" (0 for any other string).
+int ParseBisection(const std::string& ks);
+// Recursive bisection into up to 2^levels clusters (KMeans::Run2x,
+// kmeans.cc:284-383): every node splits its vertices in two with the
+// member-subset k-means, the halves oriented so the most similar
+// child-level centroids meet, ids renumbered along the final order.
+std::vector<int> GpuKMeansBisect(const Graph& g, int levels, int iters, uint64_t seed, int device, int* clusters);
+// "2^x" to GpuKMeansBisect, any other level string to GpuKMeansLevels.
+std::vector<int> GpuKMeansSpec(const Graph& g, const std::string& ks, int iters, uint64_t seed, int device, bool reorder, int* clusters);
 // Empty clusters removed and the rest renumbered in order
 // (RemoveEmptyClusters, kmeans.cc:491-510); returns the cluster count.
 int RemoveEmptyClusters(int k, std::vector<int>& assignment);
