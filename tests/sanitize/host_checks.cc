@@ -178,6 +178,13 @@ void CApi(const std::string& dir) {
   part[3] = 5;
   Expect(vsgm_view_pack(g, 2, part.data(), 1, 0, 1ull << 40, 1, 0, 1, assign.data(), &bins, nullptr, nullptr, 0) != 0,
          "bad partition id reported");
+  // null handles and outputs are errors, not crashes
+  Expect(vsgm_graph_write_bin(nullptr, (dir + "/n.bin").c_str()) != 0, "null graph reported");
+  Expect(vsgm_graph_read_bin(nullptr, &r) != 0 && vsgm_graph_read_bin((dir + "/c.bin").c_str(), nullptr) != 0, "null path / out reported");
+  Expect(vsgm_plan_optimize(nullptr, &p) != 0 && vsgm_query_preset(1, nullptr) != 0, "null query / out reported");
+  Expect(vsgm_executor_match(nullptr, 1) != 0 && vsgm_executor_counts(nullptr, nullptr, 0) != 0, "null executor reported");
+  int n_graphlets = 0;
+  Expect(vsgm_query_graphlets(4, nullptr, 0, &n_graphlets) == 0 && n_graphlets == 6, "graphlet count without storage");
   vsgm_plan_free(p);
   vsgm_query_free(q);
   vsgm_graph_free(g);

@@ -57,15 +57,22 @@ extern "C" {
 const char* vsgm_error(void) { return g_error.c_str(); }
 
 int vsgm_graph_read_bin(const char* path, vsgm_graph** out) {
-  return Guard([&] { *out = new vsgm_graph{vsgm::ReadCSRBinFile(path)}; });
+  return Guard([&] {
+    VSGM_CHECK(path && out, "null argument");
+    *out = new vsgm_graph{vsgm::ReadCSRBinFile(path)};
+  });
 }
 
 int vsgm_graph_write_bin(const vsgm_graph* g, const char* path) {
-  return Guard([&] { vsgm::WriteCSRBinFile(path, g->g); });
+  return Guard([&] {
+    VSGM_CHECK(g && path, "null argument");
+    vsgm::WriteCSRBinFile(path, g->g);
+  });
 }
 
 int vsgm_graph_from_csr(uint64_t vertex_count, const uint64_t* row_ptrs, const uint32_t* cols, vsgm_graph** out) {
   return Guard([&] {
+    VSGM_CHECK(out, "null argument");
     auto* h = new vsgm_graph;
     h->g.vertex_count = vertex_count;
     h->g.row_ptrs.assign(row_ptrs, row_ptrs + vertex_count + 1);
@@ -76,6 +83,7 @@ int vsgm_graph_from_csr(uint64_t vertex_count, const uint64_t* row_ptrs, const u
 
 int vsgm_graph_from_edges(uint64_t vertex_count, const uint32_t* pairs, uint64_t n_pairs, vsgm_graph** out) {
   return Guard([&] {
+    VSGM_CHECK(out, "null argument");
     std::vector<std::pair<vsgm::uintV, vsgm::uintV>> edges(n_pairs);
     for (uint64_t i = 0; i < n_pairs; ++i) {
       VSGM_CHECK(pairs[2 * i] < vertex_count && pairs[2 * i + 1] < vertex_count, "edge endpoint out of range");
@@ -86,27 +94,43 @@ int vsgm_graph_from_edges(uint64_t vertex_count, const uint32_t* pairs, uint64_t
 }
 
 int vsgm_graph_read_snap(const char* path, int directed, int device, int threads, vsgm_graph** out) {
-  return Guard([&] { *out = new vsgm_graph{vsgm::ReadSNAPFile(path, directed != 0, device, threads)}; });
+  return Guard([&] {
+    VSGM_CHECK(path && out, "null argument");
+    *out = new vsgm_graph{vsgm::ReadSNAPFile(path, directed != 0, device, threads)};
+  });
 }
 
 int vsgm_graph_load(const char* path, int device, int threads, int drop_isolated, vsgm_graph** out) {
-  return Guard([&] { *out = new vsgm_graph{vsgm::LoadGraph(path, device, threads, drop_isolated != 0)}; });
+  return Guard([&] {
+    VSGM_CHECK(path && out, "null argument");
+    *out = new vsgm_graph{vsgm::LoadGraph(path, device, threads, drop_isolated != 0)};
+  });
 }
 
 int vsgm_graph_drop_isolated(const vsgm_graph* g, vsgm_graph** out) {
-  return Guard([&] { *out = new vsgm_graph{vsgm::DropIsolated(g->g)}; });
+  return Guard([&] {
+    VSGM_CHECK(g && out, "null argument");
+    *out = new vsgm_graph{vsgm::DropIsolated(g->g)};
+  });
 }
 
 int vsgm_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads, vsgm_graph** out) {
-  return Guard([&] { *out = new vsgm_graph{vsgm::GenerateRMAT(scale, edge_factor, a, b, c, seed, threads)}; });
+  return Guard([&] {
+    VSGM_CHECK(out, "null argument");
+    *out = new vsgm_graph{vsgm::GenerateRMAT(scale, edge_factor, a, b, c, seed, threads)};
+  });
 }
 
 int vsgm_graph_chung_lu(uint64_t vertex_count, uint64_t draws, double gamma, uint64_t seed, int threads, vsgm_graph** out) {
-  return Guard([&] { *out = new vsgm_graph{vsgm::GenerateChungLu(vertex_count, draws, gamma, seed, threads)}; });
+  return Guard([&] {
+    VSGM_CHECK(out, "null argument");
+    *out = new vsgm_graph{vsgm::GenerateChungLu(vertex_count, draws, gamma, seed, threads)};
+  });
 }
 
 int vsgm_graph_info(const vsgm_graph* g, uint64_t* vertex_count, uint64_t* entries, uint64_t* max_degree) {
   return Guard([&] {
+    VSGM_CHECK(g, "null argument");
     if (vertex_count) *vertex_count = g->g.vertex_count;
     if (entries) *entries = g->g.cols.size();
     if (max_degree) *max_degree = g->g.MaxDegree();
@@ -115,6 +139,7 @@ int vsgm_graph_info(const vsgm_graph* g, uint64_t* vertex_count, uint64_t* entri
 
 int vsgm_graph_csr(const vsgm_graph* g, const uint64_t** row_ptrs, const uint32_t** cols) {
   return Guard([&] {
+    VSGM_CHECK(g && row_ptrs && cols, "null argument");
     *row_ptrs = g->g.row_ptrs.data();
     *cols = g->g.cols.data();
   });
@@ -122,6 +147,7 @@ int vsgm_graph_csr(const vsgm_graph* g, const uint64_t** row_ptrs, const uint32_
 
 int vsgm_graph_validate(const vsgm_graph* g, char* msg, size_t len) {
   return Guard([&] {
+    VSGM_CHECK(g, "null argument");
     std::string m = g->g.Validate(true);
     if (len) {
       strncpy(msg, m.c_str(), len - 1);
@@ -133,15 +159,22 @@ int vsgm_graph_validate(const vsgm_graph* g, char* msg, size_t len) {
 void vsgm_graph_free(vsgm_graph* g) { delete g; }
 
 int vsgm_query_preset(int preset, vsgm_query** out) {
-  return Guard([&] { *out = new vsgm_query{vsgm::Query::Preset(preset)}; });
+  return Guard([&] {
+    VSGM_CHECK(out, "null argument");
+    *out = new vsgm_query{vsgm::Query::Preset(preset)};
+  });
 }
 
 int vsgm_query_clique(int k, vsgm_query** out) {
-  return Guard([&] { *out = new vsgm_query{vsgm::Query::Clique(k)}; });
+  return Guard([&] {
+    VSGM_CHECK(out, "null argument");
+    *out = new vsgm_query{vsgm::Query::Clique(k)};
+  });
 }
 
 int vsgm_query_from_matrix(int n, const uint8_t* adj, vsgm_query** out) {
   return Guard([&] {
+    VSGM_CHECK(out, "null argument");
     std::vector<std::vector<int>> lists(n);
     for (int i = 0; i < n; ++i)
       for (int j = 0; j < n; ++j)
@@ -155,6 +188,7 @@ int vsgm_query_from_matrix(int n, const uint8_t* adj, vsgm_query** out) {
 
 int vsgm_query_graphlets(int k, vsgm_query** out, int cap, int* n_out) {
   return Guard([&] {
+    VSGM_CHECK(n_out && (out || cap <= 0), "null argument");
     auto qs = vsgm::Query::Graphlets(k);
     *n_out = (int)qs.size();
     for (int i = 0; i < (int)qs.size() && i < cap; ++i) out[i] = new vsgm_query{qs[i]};
@@ -163,6 +197,7 @@ int vsgm_query_graphlets(int k, vsgm_query** out, int cap, int* n_out) {
 
 int vsgm_query_info(const vsgm_query* q, int* n, int* edges, uint8_t* adj, uint8_t* cond) {
   return Guard([&] {
+    VSGM_CHECK(q, "null argument");
     int k = (int)q->q.GetVertexCount();
     if (n) *n = k;
     if (edges) *edges = (int)q->q.GetEdgeCount();
@@ -181,6 +216,7 @@ void vsgm_query_free(vsgm_query* q) { delete q; }
 
 int vsgm_plan_optimize(const vsgm_query* q, vsgm_plan** out) {
   return Guard([&] {
+    VSGM_CHECK(q && out, "null argument");
     auto p = std::make_unique<vsgm::Plan>(q->q);
     p->Optimize();
     *out = new vsgm_plan{std::move(p)};
@@ -189,6 +225,7 @@ int vsgm_plan_optimize(const vsgm_query* q, vsgm_plan** out) {
 
 int vsgm_plan_from_order(const vsgm_query* q, const uint32_t* seq, int n, vsgm_plan** out) {
   return Guard([&] {
+    VSGM_CHECK(q && out, "null argument");
     std::vector<vsgm::uintV> s(seq, seq + n);
     *out = new vsgm_plan{std::make_unique<vsgm::Plan>(vsgm::Plan::FromOrder(q->q, s))};
   });
@@ -196,6 +233,7 @@ int vsgm_plan_from_order(const vsgm_query* q, const uint32_t* seq, int n, vsgm_p
 
 int vsgm_plan_info(const vsgm_plan* p, int* hop, uint64_t* root_degree, int* n_ops) {
   return Guard([&] {
+    VSGM_CHECK(p, "null argument");
     if (hop) *hop = p->p->GetHop();
     if (root_degree) *root_degree = p->p->GetRootDegree();
     if (n_ops) *n_ops = (int)p->p->GetExecuteOperations().size();
@@ -204,6 +242,7 @@ int vsgm_plan_info(const vsgm_plan* p, int* hop, uint64_t* root_degree, int* n_o
 
 int vsgm_plan_op(const vsgm_plan* p, int i, int* op, int* vertex) {
   return Guard([&] {
+    VSGM_CHECK(p, "null argument");
     const auto& ops = p->p->GetExecuteOperations();
     VSGM_CHECK(i >= 0 && i < (int)ops.size(), "operation index out of range");
     *op = (int)ops[i].first;
@@ -213,6 +252,7 @@ int vsgm_plan_op(const vsgm_plan* p, int i, int* op, int* vertex) {
 
 int vsgm_plan_order(const vsgm_plan* p, uint32_t* seq, int cap) {
   return Guard([&] {
+    VSGM_CHECK(p, "null argument");
     const auto& s = p->p->GetSearchSequence();
     VSGM_CHECK(cap >= (int)s.size(), "order buffer too small");
     for (size_t i = 0; i < s.size(); ++i) seq[i] = s[i];
@@ -221,6 +261,7 @@ int vsgm_plan_order(const vsgm_plan* p, uint32_t* seq, int cap) {
 
 int vsgm_plan_serialize(const vsgm_plan* p, char* buf, size_t len, size_t* needed) {
   return Guard([&] {
+    VSGM_CHECK(p, "null argument");
     std::string s = p->p->Serialize();
     if (needed) *needed = s.size() + 1;
     if (buf && len) {
@@ -237,6 +278,7 @@ int vsgm_view_pack(const vsgm_graph* g, int hop, const int32_t* partition, int f
                    int pool_size, int degree_order, int threads, int32_t* out_assign, int* out_bins, uint64_t* out_sources,
                    uint64_t* out_sizes, int cap) {
   return Guard([&] {
+    VSGM_CHECK(g, "null argument");
     VSGM_CHECK(g && partition && out_assign && out_bins, "null argument");
     const uint64_t V = g->g.vertex_count;
     vsgm::PackedBins p = vsgm::PackViewBins(g->g, hop, std::vector<int>(partition, partition + V), first_layer, second_layer,
@@ -252,6 +294,7 @@ int vsgm_view_pack(const vsgm_graph* g, int hop, const int32_t* partition, int f
 int vsgm_graph_kmeans(const vsgm_graph* g, const char* levels, int iters, uint64_t seed, int device, int reorder, int32_t* assignment,
                       int* clusters) {
   return Guard([&] {
+    VSGM_CHECK(g && levels, "null argument");
     VSGM_CHECK(g && levels && assignment && clusters && iters >= 0, "bad arguments");
     std::vector<int> a = vsgm::GpuKMeansSpec(g->g, levels, iters, seed, device, reorder != 0, clusters);
     std::copy(a.begin(), a.end(), assignment);
@@ -260,6 +303,7 @@ int vsgm_graph_kmeans(const vsgm_graph* g, const char* levels, int iters, uint64
 
 int vsgm_graph_fennel(const vsgm_graph* g, int parts, int hop, int32_t* assignment) {
   return Guard([&] {
+    VSGM_CHECK(g, "null argument");
     VSGM_CHECK(g && assignment, "bad arguments");
     std::vector<int> a = vsgm::FennelPartition(g->g, parts, hop);
     std::copy(a.begin(), a.end(), assignment);
@@ -268,6 +312,7 @@ int vsgm_graph_fennel(const vsgm_graph* g, int parts, int hop, int32_t* assignme
 
 int vsgm_clusters_reorder(const vsgm_graph* g, int k, int32_t* assignment) {
   return Guard([&] {
+    VSGM_CHECK(g, "null argument");
     VSGM_CHECK(g && assignment && k >= 1, "bad arguments");
     const uint64_t V = g->g.vertex_count;
     std::vector<int> a(assignment, assignment + V);
@@ -288,11 +333,15 @@ int vsgm_clusters_compact(int k, int32_t* assignment, uint64_t vertex_count, int
 }
 
 int vsgm_vbmap_write(const char* path, int n_bins, const int* assignment, uint64_t vertex_count) {
-  return Guard([&] { vsgm::WriteViewBinPartition(path, n_bins, std::vector<int>(assignment, assignment + vertex_count)); });
+  return Guard([&] {
+    VSGM_CHECK(path, "null argument");
+    vsgm::WriteViewBinPartition(path, n_bins, std::vector<int>(assignment, assignment + vertex_count));
+  });
 }
 
 int vsgm_view_manager_load(const vsgm_graph* g, int hop, int threads, const char* vbmap, vsgm_view_manager** out) {
   return Guard([&] {
+    VSGM_CHECK(g && vbmap && out, "null argument");
     auto m = std::make_unique<vsgm::ViewBinManager>(&g->g, hop, threads);
     m->LoadViewBinPartition(vbmap);
     *out = new vsgm_view_manager{std::move(m)};
@@ -302,6 +351,7 @@ int vsgm_view_manager_load(const vsgm_graph* g, int hop, int threads, const char
 int vsgm_view_manager_from_assignment(const vsgm_graph* g, int hop, int threads, int n_bins, const int* assignment,
                                       vsgm_view_manager** out) {
   return Guard([&] {
+    VSGM_CHECK(g && out, "null argument");
     auto m = std::make_unique<vsgm::ViewBinManager>(&g->g, hop, threads);
     m->SetPartition(n_bins, std::vector<int>(assignment, assignment + g->g.vertex_count));
     *out = new vsgm_view_manager{std::move(m)};
@@ -310,27 +360,38 @@ int vsgm_view_manager_from_assignment(const vsgm_graph* g, int hop, int threads,
 
 int vsgm_view_manager_info(const vsgm_view_manager* m, int* n_bins, uint64_t* max_sources) {
   return Guard([&] {
+    VSGM_CHECK(m, "null argument");
     if (n_bins) *n_bins = m->m->GetViewBinNumber();
     if (max_sources) *max_sources = m->m->GetMaxPartitionedSourcesNum();
   });
 }
 
 int vsgm_view_manager_reorder(vsgm_view_manager* m) {
-  return Guard([&] { m->m->Reorder(); });
+  return Guard([&] {
+    VSGM_CHECK(m, "null argument");
+    m->m->Reorder();
+  });
 }
 
 int vsgm_view_manager_split(vsgm_view_manager* m, int devices, int times) {
-  return Guard([&] { m->m->Split(devices, times); });
+  return Guard([&] {
+    VSGM_CHECK(m, "null argument");
+    m->m->Split(devices, times);
+  });
 }
 
 void vsgm_view_manager_free(vsgm_view_manager* m) { delete m; }
 
 int vsgm_view_holder_create(uint64_t vertex_count, uint64_t max_bytes, vsgm_view_holder** out) {
-  return Guard([&] { *out = new vsgm_view_holder{std::make_unique<vsgm::ViewBinHolder>(vertex_count, max_bytes)}; });
+  return Guard([&] {
+    VSGM_CHECK(out, "null argument");
+    *out = new vsgm_view_holder{std::make_unique<vsgm::ViewBinHolder>(vertex_count, max_bytes)};
+  });
 }
 
 int vsgm_view_materialize(const vsgm_view_manager* m, int index, vsgm_view_holder* h, uint64_t root_degree) {
   return Guard([&] {
+    VSGM_CHECK(m && h, "null argument");
     const auto& pool = m->m->GetViewBinPool();
     VSGM_CHECK(index >= 0 && index < (int)pool.size(), "view bin index out of range");
     pool[index]->Materialize(*h->h, root_degree);
@@ -339,6 +400,7 @@ int vsgm_view_materialize(const vsgm_view_manager* m, int index, vsgm_view_holde
 
 int vsgm_view_holder_info(const vsgm_view_holder* h, int* bin_id, uint64_t* sources, uint64_t* entries, uint64_t* vertices) {
   return Guard([&] {
+    VSGM_CHECK(h, "null argument");
     if (bin_id) *bin_id = h->h->GetId();
     if (sources) *sources = h->h->GetSourcesNum();
     if (entries) *entries = h->h->GetTotalSize();
@@ -348,6 +410,7 @@ int vsgm_view_holder_info(const vsgm_view_holder* h, int* bin_id, uint64_t* sour
 
 int vsgm_view_holder_csr(const vsgm_view_holder* h, const uint64_t** row_ptrs, const uint32_t** cols, const uint32_t** sources) {
   return Guard([&] {
+    VSGM_CHECK(h && row_ptrs && cols && sources, "null argument");
     if (row_ptrs) *row_ptrs = h->h->GetRowPtrs();
     if (cols) *cols = h->h->GetCols();
     if (sources) *sources = h->h->GetSources().data();
@@ -377,6 +440,7 @@ int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* con
 
 int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
   return Guard([&] {
+    VSGM_CHECK(e && key, "null argument");
     auto& o = e->e->Options();
     std::string k = key;
     if (k == "batch_bytes") o.batch_bytes = (uint64_t)value;
@@ -403,6 +467,7 @@ int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
 
 int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_kernel, uint32_t max_grid, uint64_t batch_bytes) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     if (isect_kernel > 2 || max_grid == 0 || max_grid > (1u << 20)) throw vsgm::Error("bad plan tuning");
     e->e->SetPlanTuning((size_t)plan, vsgm::PlanTuning{isect_kernel, max_grid, batch_bytes});
   });
@@ -410,6 +475,7 @@ int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_ker
 
 int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* isect_kernel, uint32_t* max_grid, uint64_t* batch_bytes) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     const auto& t = e->e->GetPlanTuning((size_t)plan);
     *isect_kernel = t.isect_kernel;
     *max_grid = t.max_grid;
@@ -419,17 +485,22 @@ int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* is
 
 int vsgm_executor_add_view(vsgm_executor* e, const vsgm_view_holder* h, int* index) {
   return Guard([&] {
+    VSGM_CHECK(e && h, "null argument");
     size_t i = e->e->AddView(*h->h);
     if (index) *index = (int)i;
   });
 }
 
 int vsgm_executor_set_graph(vsgm_executor* e, const vsgm_graph* g) {
-  return Guard([&] { e->e->SetGraph(g->g); });
+  return Guard([&] {
+    VSGM_CHECK(e && g, "null argument");
+    e->e->SetGraph(g->g);
+  });
 }
 
 int vsgm_executor_transfer_device(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     VSGM_CHECK(e && (sources || n_sources == 0), "bad arguments");
     e->e->TransferDevice(std::vector<vsgm::uintV>(sources, sources + n_sources), hop, root_degree);
   });
@@ -438,6 +509,7 @@ int vsgm_executor_transfer_device(vsgm_executor* e, const uint32_t* sources, uin
 int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree,
                                   int* index) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     std::vector<vsgm::uintV> s(sources, sources + n_sources);
     size_t i = e->e->AddDeviceView(s, hop, root_degree);
     if (index) *index = (int)i;
@@ -446,6 +518,7 @@ int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uin
 
 int vsgm_executor_view_csr(const vsgm_executor* e, int view, uint64_t* row_ptrs, uint32_t* cols, uint64_t cols_cap, uint64_t* entries) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     std::vector<uint64_t> rp;
     std::vector<uint32_t> cl;
     e->e->ViewCsr((size_t)view, rp, cl);
@@ -459,11 +532,15 @@ int vsgm_executor_view_csr(const vsgm_executor* e, int view, uint64_t* row_ptrs,
 }
 
 int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans) {
-  return Guard([&] { e->e->MatchAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{}); });
+  return Guard([&] {
+    VSGM_CHECK(e, "null argument");
+    e->e->MatchAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{});
+  });
 }
 
 int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint64_t* n_nodes) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     uint64_t n = e->e->CaptureAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{});
     if (n_nodes) *n_nodes = n;
   });
@@ -471,6 +548,7 @@ int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint6
 
 int vsgm_executor_replay(vsgm_executor* e, double* ms) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     const double before = e->e->ReplayMs();
     e->e->ReplayGraph();
     if (ms) *ms = e->e->ReplayMs() - before;
@@ -478,15 +556,22 @@ int vsgm_executor_replay(vsgm_executor* e, double* ms) {
 }
 
 int vsgm_executor_hub_count(const vsgm_executor* e, uint32_t* n) {
-  return Guard([&] { *n = e->e->HubCount(); });
+  return Guard([&] {
+    VSGM_CHECK(e, "null argument");
+    *n = e->e->HubCount();
+  });
 }
 
 int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h) {
-  return Guard([&] { e->e->Transfer(*h->h); });
+  return Guard([&] {
+    VSGM_CHECK(e && h, "null argument");
+    e->e->Transfer(*h->h);
+  });
 }
 
 int vsgm_executor_transfer_graph(vsgm_executor* e, const vsgm_graph* g, const uint32_t* sources, uint64_t n_sources) {
   return Guard([&] {
+    VSGM_CHECK(e && g, "null argument");
     std::vector<vsgm::uintV> s;
     if (sources) s.assign(sources, sources + n_sources);
     e->e->TransferGraph(g->g, s);
@@ -494,11 +579,15 @@ int vsgm_executor_transfer_graph(vsgm_executor* e, const vsgm_graph* g, const ui
 }
 
 int vsgm_executor_match(vsgm_executor* e, int do_match) {
-  return Guard([&] { e->e->Match(do_match != 0); });
+  return Guard([&] {
+    VSGM_CHECK(e, "null argument");
+    e->e->Match(do_match != 0);
+  });
 }
 
 int vsgm_executor_counts(const vsgm_executor* e, uint64_t* counts, int n) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     const auto& c = e->e->GetTotalCounts();
     VSGM_CHECK(n >= (int)c.size(), "counts buffer too small");
     for (size_t i = 0; i < c.size(); ++i) counts[i] = c[i];
@@ -507,6 +596,7 @@ int vsgm_executor_counts(const vsgm_executor* e, uint64_t* counts, int n) {
 
 int vsgm_executor_match_ms(const vsgm_executor* e, double* ms, int n) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     const auto& c = e->e->GetMatchMs();
     VSGM_CHECK(n >= (int)c.size(), "buffer too small");
     for (size_t i = 0; i < c.size(); ++i) ms[i] = c[i];
@@ -514,11 +604,15 @@ int vsgm_executor_match_ms(const vsgm_executor* e, double* ms, int n) {
 }
 
 int vsgm_executor_reset(vsgm_executor* e) {
-  return Guard([&] { e->e->ResetCounts(); });
+  return Guard([&] {
+    VSGM_CHECK(e, "null argument");
+    e->e->ResetCounts();
+  });
 }
 
 int vsgm_executor_level_stats(const vsgm_executor* e, int plan, vsgm_level_stat* out, int cap, int* n_out) {
   return Guard([&] {
+    VSGM_CHECK(e, "null argument");
     const auto& all = e->e->GetLevelStats();
     VSGM_CHECK(plan >= 0 && plan < (int)all.size(), "plan index out of range");
     const auto& s = all[plan];
@@ -528,11 +622,17 @@ int vsgm_executor_level_stats(const vsgm_executor* e, int plan, vsgm_level_stat*
 }
 
 int vsgm_executor_context(vsgm_executor* e, vsgm_ctx** ctx) {
-  return Guard([&] { *ctx = e->e->Context(); });
+  return Guard([&] {
+    VSGM_CHECK(e && ctx, "null argument");
+    *ctx = e->e->Context();
+  });
 }
 
 int vsgm_executor_peak_bytes(const vsgm_executor* e, uint64_t* bytes) {
-  return Guard([&] { *bytes = e->e->PeakDeviceBytes(); });
+  return Guard([&] {
+    VSGM_CHECK(e, "null argument");
+    *bytes = e->e->PeakDeviceBytes();
+  });
 }
 
 void vsgm_executor_free(vsgm_executor* e) { delete e; }
