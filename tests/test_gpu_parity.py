@@ -285,3 +285,35 @@ def test_kmeans_tool_reads_features_graphs_as_the_reference_does(vsgm, gpu, tmp_
     closed = np.loadtxt(path + ".1hop.bin.kmeans.4", dtype=np.int64)
     # float64 atomics sum in any order: equal up to near-ties
     assert len(plain) == g.vertex_count and np.mean(plain == closed) > 0.99
+
+
+def test_pm_pipeline_checkpoint_resume(vsgm, oracle, gpu, tmp_path):
+    """-ck FILE: each finished bin appends its counts; a rerun skips the
+    bins listed (if their sources still match) and totals stay exact."""
+    g = vsgm.Graph.rmat(10, 16, seed=14)
+    path = str(tmp_path / "g.bin")
+    g.write_bin(path)
+    vsgm.ViewBinManager.write_vbmap(path + ".2.hop.vbmap", 4, np.random.default_rng(1).integers(0, 4, size=g.vertex_count))
+    exe = os.path.join(ROOT, "vsgm_amd", "bin", "pm_pipeline")
+    ck = str(tmp_path / "ck.txt")
+    want = oracle_counts(oracle, g, [vsgm.Plan.optimize(vsgm.Query.clique(4))])[0]
+
+    def run(*extra):
+        print("[gpu] pm_pipeline -ck", *extra, flush=True)
+        out = subprocess.run([exe, "-f", path, "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4", "-kc", "4", "-ck", ck] + list(extra),
+                             capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr[-2000:]
+        line = [x for x in out.stdout.splitlines() if x.startswith("total count for query 0:")][0]
+        return int(line.split(":")[1]), out.stdout
+
+    total, _ = run()
+    assert total == want and len(open(ck).read().splitlines()) == 4
+    total, out = run()
+    assert total == want and "4 of 4 bins already counted" in out
+    lines = open(ck).read().splitlines()
+    bad = lines[2].split()
+    bad[1] = str(int(bad[1]) ^ 1)  # a bin whose sources no longer match is counted again
+    open(ck, "w").write("\n".join(lines[:2] + [" ".join(bad)]) + "\n")
+    total, out = run("-dv", "1")
+    assert total == want and "2 of 4 bins already counted" in out
+    assert len(open(ck).read().splitlines()) == 5

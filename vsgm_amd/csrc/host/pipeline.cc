@@ -8,7 +8,9 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <queue>
@@ -75,6 +77,11 @@ int Run(int argc, char** argv) {
   // -gp P packs the k-means clusters into bins of at most -m GB of list
   // entries with P bins open per cluster (the view_packing tool's step).
   const int gpu_pack = fl.i("-gp", 0);
+  // -ck FILE checkpoints per-bin counts: one line "bin fingerprint n c0 .. c(n-1)"
+  // is appended as each bin finishes; a later run with the same FILE skips
+  // the bins listed there (when the bin's fingerprint, a hash of its
+  // sources, still matches) and adds their counts.
+  const std::string ck_file = fl.str("-ck", "");
 
   int gpus = 0;
   vsgm_device_count(&gpus);
@@ -140,6 +147,45 @@ int Run(int argc, char** argv) {
   BoundedQueue assigned(queue_size), released(queue_size);
   for (int i = 0; i < queue_size; ++i) released.push(i);
 
+  const auto& pool = vbm.GetViewBinPool();
+  auto fingerprint = [&](int bin) {
+    uint64_t h = 1469598103934665603ull ^ pool[bin]->GetSources().size();
+    for (uintV v : pool[bin]->GetSources()) h = (h ^ v) * 1099511628211ull;
+    return h;
+  };
+  std::map<int, std::vector<uintC>> done;
+  if (!ck_file.empty()) {
+    std::ifstream in(ck_file);
+    std::string line;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      long bin = -1;
+      uint64_t fp = 0;
+      size_t n = 0;
+      if (!(ls >> bin >> fp >> n) || bin < 0 || bin >= (long)pool.size() || n != plans.size() || fp != fingerprint((int)bin)) continue;
+      std::vector<uintC> c(n);
+      bool ok = true;
+      for (auto& x : c) ok = ok && (bool)(ls >> x);
+      if (ok) done[(int)bin] = c;
+    }
+    std::cout << "checkpoint " << ck_file << ": " << done.size() << " of " << pool.size() << " bins already counted" << std::endl;
+  }
+  std::ofstream ck;
+  if (!ck_file.empty()) {
+    ck.open(ck_file, std::ios::app);
+    if (!ck) throw Error("cannot append to " + ck_file);
+  }
+  std::mutex ck_mu;
+  auto record = [&](int bin, const std::vector<uintC>& before, const std::vector<uintC>& after) {
+    if (ck_file.empty() || !do_match) return;  // a run without matching counts nothing to keep
+    std::lock_guard<std::mutex> lk(ck_mu);
+    ck << bin << " " << fingerprint(bin) << " " << after.size();
+    for (size_t i = 0; i < after.size(); ++i) ck << " " << after[i] - before[i];
+    ck << "\n" << std::flush;
+    if (!ck) throw Error("write to " + ck_file + " failed");
+  };
+  std::vector<int> holder_bin(queue_size, -1);
+
   std::atomic<int> next_bin{0};
   std::atomic<int> producers_done{0};
   std::atomic<bool> failed{false};
@@ -149,11 +195,11 @@ int Run(int argc, char** argv) {
     std::cout << s << std::flush;
   };
   std::vector<std::thread> threads;
-  const auto& pool = vbm.GetViewBinPool();
   for (int p = 0; p < (device_views ? 0 : producers); ++p)
     threads.emplace_back([&, p] {
       try {
         for (int bin; (bin = next_bin++) < (int)pool.size() && !failed;) {
+          if (done.count(bin)) continue;
           int holder = -1;
           if (!released.pop(holder)) break;
           auto t = std::chrono::steady_clock::now();
@@ -161,6 +207,7 @@ int Run(int argc, char** argv) {
           std::ostringstream os;
           os << "producer " << p << " produces " << bin << " in " << holder << " (" << Ms(t) << " ms)\n";
           say(os.str());
+          holder_bin[holder] = bin;  // published to the consumer by the queue's lock
           assigned.push(holder);
         }
       } catch (const std::exception& ex) {
@@ -180,13 +227,16 @@ int Run(int argc, char** argv) {
         if (device_views) {
           exec.SetGraph(graph);
           for (int bin; (bin = next_bin++) < (int)pool.size() && !failed;) {
+            if (done.count(bin)) continue;
             auto t = std::chrono::steady_clock::now();
             exec.TransferDevice(pool[bin]->GetSources(), hop, root_degree);
             std::ostringstream os;
             os << "consumer " << c << " builds view " << bin << " (" << Ms(t) << " ms)\n";
             say(os.str());
             t = std::chrono::steady_clock::now();
+            const std::vector<uintC> before = exec.GetTotalCounts();
             exec.Match(do_match != 0);
+            record(bin, before, exec.GetTotalCounts());
             std::ostringstream os2;
             os2 << "consumer " << c << " completes match of view " << bin << " (" << Ms(t) << " ms)\n";
             say(os2.str());
@@ -201,9 +251,12 @@ int Run(int argc, char** argv) {
           std::ostringstream os;
           os << "consumer " << c << " transfers holder " << holder << " (" << Ms(t) << " ms)\n";
           say(os.str());
+          const int bin = holder_bin[holder];
           released.push(holder);
           t = std::chrono::steady_clock::now();
+          const std::vector<uintC> before = exec.GetTotalCounts();
           exec.Match(do_match != 0);
+          record(bin, before, exec.GetTotalCounts());
           std::ostringstream os2;
           os2 << "consumer " << c << " completes match of holder " << holder << " (" << Ms(t) << " ms)\n";
           say(os2.str());
@@ -223,6 +276,7 @@ int Run(int argc, char** argv) {
   for (size_t i = 0; i < plans.size(); ++i) {
     uintC total = 0;
     for (int c = 0; c < consumers; ++c) total += counts[c][i];
+    for (auto& d : done) total += d.second[i];
     std::cout << "total count for query " << i << ": " << total << std::endl;
   }
   return 0;
