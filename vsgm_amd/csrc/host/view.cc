@@ -71,7 +71,7 @@ void ViewBin::Materialize(ViewBinHolder& h, uint64_t root_degree) const {
         uintV v = frontier[i];
         for (uintE j = graph_->row_ptrs[v]; j < graph_->row_ptrs[v + 1]; ++j) {
           uintV w = graph_->cols[j];
-          if (!added[w]) {
+          if (!__atomic_load_n(&added[w], __ATOMIC_RELAXED)) {  // other threads set marks concurrently
             char expected = 0;
             if (__atomic_compare_exchange_n(&added[w], &expected, (char)1, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) mine.push_back(w);
           }
