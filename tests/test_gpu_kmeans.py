@@ -162,3 +162,19 @@ def test_multi_level_nests_and_keeps_cliques_whole(vsgm, gpu):
     assert all(len(set(b[c * m:(c + 1) * m].tolist())) == 1 for c in range(n))
     with pytest.raises(vsgm.VsgmError):
         vsgm.kmeans_levels(g, "2^0")
+
+
+def test_symmetric_graphs_partition_the_same_every_run(vsgm, gpu):
+    """On a symmetric graph the update gathers in a fixed order and the
+    norms fold in wave order: two runs with one seed give the same partition
+    bit for bit (whole runs and member runs)."""
+    from vsgm_amd.partition import kmeans, kmeans_members
+
+    g = vsgm.Graph.rmat(12, 8, seed=9)
+    a1, s1, _ = kmeans(g, 6, iters=5, seed=3)
+    a2, s2, _ = kmeans(g, 6, iters=5, seed=3)
+    assert np.array_equal(a1, a2) and np.array_equal(s1, s2)
+    members = np.arange(0, g.vertex_count, 3)
+    m1, _, _ = kmeans_members(g, members, 4, iters=5, seed=8)
+    m2, _, _ = kmeans_members(g, members, 4, iters=5, seed=8)
+    assert np.array_equal(m1, m2)

@@ -2,9 +2,12 @@
 // gfx950 kernels. Each pass is a sweep over the CSR: one wavefront per
 // vertex, lanes striding its list. Centroids are double [V][K]: the K
 // values a list entry touches share one cache line, so an assignment pass
-// reads each list once and one line per entry; the centroid update
-// scatters 1/sqrt(|N[v]|) into the member's column with hardware float64
-// atomics (no compare-and-swap loops) and then normalises by cluster size.
+// reads each list once and one line per entry. On symmetric graphs the
+// centroid update gathers (dimension w sums its closed neighbours' shares,
+// one wavefront per w, fixed order) and the column norms are folded from
+// per-wave partials in wave order, so a seed fixes the partition bit for
+// bit; other graphs scatter 1/sqrt(|N[v]|) into the member's column with
+// hardware float64 atomics. Both then normalise by cluster size.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -136,10 +139,12 @@ __global__ __launch_bounds__(kBlock) void k_km_sizes(const int32_t* __restrict__
   if (threadIdx.x < (unsigned)K && s_hist[threadIdx.x]) atomicAdd(&sizes[threadIdx.x], s_hist[threadIdx.x]);
 }
 
-// Pass 2: divide by the cluster size, accumulate the squared norms.
+// Pass 2: divide by the cluster size; each wave writes its share of the
+// squared column norms to partials[wave][K] (folded by k_km_fold, so the
+// norms do not depend on the order waves finish in).
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_km_finish(double* __restrict__ cent, uint64_t V, const unsigned long long* __restrict__ sizes,
-                                                      double* __restrict__ sqr) {
+                                                      double* __restrict__ partials) {
   double part[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) part[k] = 0.0;
@@ -152,30 +157,89 @@ __global__ __launch_bounds__(kBlock) void k_km_finish(double* __restrict__ cent,
       part[k] += c * c;
     }
   }
+  const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const double s = wave_sum_d(part[k]);
-    if ((threadIdx.x & (kWave - 1)) == 0 && s != 0.0) unsafeAtomicAdd(&sqr[k], s);
+    if ((threadIdx.x & (kWave - 1)) == 0) partials[wave * K + k] = s;
   }
 }
 
 template <int K>
-__global__ __launch_bounds__(kBlock) void k_km_colsq(const double* __restrict__ cent, uint64_t V, double* __restrict__ sqr) {
+__global__ __launch_bounds__(kBlock) void k_km_colsq(const double* __restrict__ cent, uint64_t V, double* __restrict__ partials) {
   double part[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) part[k] = 0.0;
   for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock)
 #pragma unroll
     for (int k = 0; k < K; ++k) part[k] += cent[v * K + k] * cent[v * K + k];
+  const uint64_t wave = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const double s = wave_sum_d(part[k]);
-    if ((threadIdx.x & (kWave - 1)) == 0 && s != 0.0) unsafeAtomicAdd(&sqr[k], s);
+    if ((threadIdx.x & (kWave - 1)) == 0) partials[wave * K + k] = s;
   }
 }
 
-__global__ void k_km_sqrt(double* d, int n) {
-  if ((int)threadIdx.x < n) d[threadIdx.x] = sqrt(d[threadIdx.x]);
+// out[k] = sqrt(sum over waves of partials[wave][k]), summed in wave order.
+__global__ void k_km_fold(const double* __restrict__ partials, uint64_t waves, int K, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= K) return;
+  double s = 0.0;
+  for (uint64_t w = 0; w < waves; ++w) s += partials[w * K + k];
+  out[k] = sqrt(s);
+}
+
+// Centroid update for symmetric graphs as a gather: dimension w of
+// centroid k sums 1/sqrt(|N[v]|) over the members v of k in N[w] (the
+// vertices whose closed neighbourhood holds w), one wavefront per w, in a
+// fixed order; writes every entry, so no atomics and no zeroing.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_km_gather_cent(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols, uint64_t V,
+                                                           const int32_t* __restrict__ assign, double* __restrict__ cent) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; w < V; w += (uint64_t)gridDim.x * kWavesPerBlock) {
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    auto add = [&](uint64_t v) {
+      const int k = assign[v];
+      const double r = sqrt(1.0 / (double)(rp[v + 1] - rp[v] + 1));
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) acc[kk] += kk == k ? r : 0.0;
+    };
+    for (uint64_t j = rp[w] + lane; j < rp[w + 1]; j += kWave) add(cols[j]);
+    if (lane == 0) add(w);
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = wave_sum_d(acc[k]);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) cent[w * K + k] = acc[k];
+    }
+  }
+}
+
+// Counts list entries (v, w) whose reverse entry (w, v) is missing; rows
+// must be sorted (an unsorted row only makes the graph look asymmetric).
+__global__ __launch_bounds__(kBlock) void k_km_asym(const uint64_t* __restrict__ rp, const uint32_t* __restrict__ cols, uint64_t V,
+                                                    unsigned long long* __restrict__ missing) {
+  const int lane = threadIdx.x & (kWave - 1);
+  unsigned long long local = 0;
+  for (uint64_t v = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; v < V; v += (uint64_t)gridDim.x * kWavesPerBlock)
+    for (uint64_t j = rp[v] + lane; j < rp[v + 1]; j += kWave) {
+      const uint64_t w = cols[j];
+      uint64_t lo = rp[w], hi = rp[w + 1];
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (cols[mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo == rp[w + 1] || cols[lo] != v) ++local;
+    }
+  for (int o = kWave / 2; o > 0; o >>= 1) local += __shfl_xor(local, o);
+  if (lane == 0 && local) atomicAdd(missing, local);
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero_d(double* d, uint64_t n) {
@@ -286,26 +350,50 @@ int assign_pass(vsgm_ctx* ctx, const uint64_t* rp, const uint32_t* cols, uint64_
   return check(ctx, "k_km_assign launch failed");
 }
 
-int update_pass(vsgm_ctx* ctx, const uint64_t* rp, const uint32_t* cols, uint64_t V, int K, const int32_t* assign, double* cent, double* norm,
-                uint64_t* sizes) {
+// Waves of the norm kernels' grid: the rows of their partials.
+uint64_t fold_waves(uint64_t V) { return (uint64_t)grid_for((V + kBlock - 1) / kBlock) * kWavesPerBlock; }
+
+// *symmetric = every list entry (v, w) has its reverse (w, v).
+int symmetric_graph(vsgm_ctx* ctx, const uint64_t* rp, const uint32_t* cols, uint64_t V, bool* symmetric) {
   hipStream_t s = vsgm_internal::stream(ctx);
-  hipLaunchKernelGGL(k_zero_d, dim3(grid_for((V * K + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cent, V * K);
-  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(kBlock), 0, s, norm, (uint64_t)K);
+  unsigned long long* missing = nullptr;
+  int st = vsgm_alloc(ctx, 2 * sizeof(unsigned long long), (void**)&missing);  // k_set_u64 writes two words
+  if (st != VSGM_OK) return st;
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, missing, 0ull, 0ull);
+  hipLaunchKernelGGL(k_km_asym, dim3(wave_grid(V)), dim3(kBlock), 0, s, rp, cols, V, missing);
+  unsigned long long h = 1;
+  if ((st = check(ctx, "symmetry check launch failed")) == VSGM_OK) st = vsgm_d2h(ctx, &h, missing, sizeof(h));
+  vsgm_free(ctx, missing);
+  *symmetric = st == VSGM_OK && h == 0;
+  return st;
+}
+
+// Centroids and norms from an assignment. Symmetric graphs use the gather
+// form (no atomics: the same assignment gives the same centroids bit for
+// bit); others scatter with float64 atomics. partials holds fold_waves(V)
+// x K doubles.
+int update_pass(vsgm_ctx* ctx, const uint64_t* rp, const uint32_t* cols, uint64_t V, int K, const int32_t* assign, double* cent, double* norm,
+                uint64_t* sizes, bool symmetric, double* partials) {
+  hipStream_t s = vsgm_internal::stream(ctx);
+  if (!symmetric) hipLaunchKernelGGL(k_zero_d, dim3(grid_for((V * K + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cent, V * K);
   hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(kBlock), 0, s, (double*)sizes, (uint64_t)K);
   hipLaunchKernelGGL(k_km_sizes, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, assign, V, K, (unsigned long long*)sizes);
   switch (K) {
 #define X(N)                                                                                                                       \
   case N:                                                                                                                          \
-    hipLaunchKernelGGL(k_km_scatter<N>, dim3(wave_grid(V)), dim3(kBlock), 0, s, rp, cols, V, assign, cent);                        \
+    if (symmetric)                                                                                                                 \
+      hipLaunchKernelGGL(k_km_gather_cent<N>, dim3(wave_grid(V)), dim3(kBlock), 0, s, rp, cols, V, assign, cent);                  \
+    else                                                                                                                           \
+      hipLaunchKernelGGL(k_km_scatter<N>, dim3(wave_grid(V)), dim3(kBlock), 0, s, rp, cols, V, assign, cent);                      \
     hipLaunchKernelGGL(k_km_finish<N>, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cent, V,                     \
-                       (const unsigned long long*)sizes, norm);                                                                    \
+                       (const unsigned long long*)sizes, partials);                                                                \
     break;
     VSGM_KM_CASES(X)
 #undef X
     default:
       return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "k out of range");
   }
-  hipLaunchKernelGGL(k_km_sqrt, dim3(1), dim3(64), 0, s, norm, K);
+  hipLaunchKernelGGL(k_km_fold, dim3(1), dim3(64), 0, s, partials, fold_waves(V), K, norm);
   return check(ctx, "k-means update launch failed");
 }
 
@@ -330,7 +418,13 @@ int vsgm_kmeans_update(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t
   if (vertex_count && (!d_row_ptrs || !d_cols || !d_cent || !d_norm || !d_assign || !d_sizes))
     return vsgm_internal::fail_msg(ctx, VSGM_ERR_ARG, "null k-means argument");
   if (vertex_count == 0) return VSGM_OK;
-  return update_pass(ctx, d_row_ptrs, d_cols, vertex_count, k, d_assign, d_cent, d_norm, d_sizes);
+  bool symmetric = false;
+  double* partials = nullptr;
+  int st = symmetric_graph(ctx, d_row_ptrs, d_cols, vertex_count, &symmetric);
+  if (st == VSGM_OK) st = vsgm_alloc(ctx, fold_waves(vertex_count) * k * sizeof(double), (void**)&partials);
+  if (st == VSGM_OK) st = update_pass(ctx, d_row_ptrs, d_cols, vertex_count, k, d_assign, d_cent, d_norm, d_sizes, symmetric, partials);
+  if (partials) vsgm_free(ctx, partials);
+  return st;
 }
 
 }  // extern "C"
@@ -364,7 +458,11 @@ int run_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols
   if (st == VSGM_OK) st = get(sizeof(double), (void**)&obj);
   if (st == VSGM_OK) st = get(K * sizeof(uint64_t), (void**)&sizes);
   if (st == VSGM_OK) st = get(2 * sizeof(uint64_t), (void**)&keys);
+  double* partials = nullptr;
+  if (st == VSGM_OK) st = get(fold_waves(V) * K * sizeof(double), (void**)&partials);
   if (st != VSGM_OK) return done(st);
+  bool symmetric = false;
+  if ((st = symmetric_graph(ctx, d_row_ptrs, d_cols, V, &symmetric)) != VSGM_OK) return done(st);
 
   // initialisation: farthest-first from the seeded vertex
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for((V * K + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cent, V * K);
@@ -376,9 +474,8 @@ int run_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols
 #define X(N)                                                                                                    \
   case N:                                                                                                       \
     hipLaunchKernelGGL(k_km_set<N>, dim3(1), dim3(kBlock), 0, s, d_row_ptrs, d_cols, chosen, c, cent);          \
-    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(kBlock), 0, s, norm, (uint64_t)K);                                \
-    hipLaunchKernelGGL(k_km_colsq<N>, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cent, V, norm); \
-    hipLaunchKernelGGL(k_km_sqrt, dim3(1), dim3(64), 0, s, norm, K);                                            \
+    hipLaunchKernelGGL(k_km_colsq<N>, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cent, V, partials); \
+    hipLaunchKernelGGL(k_km_fold, dim3(1), dim3(64), 0, s, partials, fold_waves(V), K, norm);                  \
     if (c + 1 < K)                                                                                              \
       hipLaunchKernelGGL(k_km_pp<N>, dim3(wave_grid(V)), dim3(kBlock), 0, s, d_row_ptrs, d_cols, V, cent, norm, c, chosen, maxsim); \
     break;
@@ -403,7 +500,7 @@ int run_kmeans(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols
       hipLaunchKernelGGL(k_km_mask_i, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, assign, mask, V);
       hipLaunchKernelGGL(k_km_mask_d, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, sim, mask, V, 0.0);
     }
-    if ((st = update_pass(ctx, d_row_ptrs, d_cols, V, K, assign, cent, norm, sizes)) != VSGM_OK) return done(st);
+    if ((st = update_pass(ctx, d_row_ptrs, d_cols, V, K, assign, cent, norm, sizes, symmetric, partials)) != VSGM_OK) return done(st);
     if (h_objective) {
       hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(kBlock), 0, s, obj, (uint64_t)1);
       hipLaunchKernelGGL(k_sum_d, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, sim, V, obj);
