@@ -42,6 +42,28 @@ def make_query(V, key):
     return V.Query.clique(k) if kind == "clique" else V.Query.preset(k)
 
 
+def intersection_rates(plan_text, ops, levels, seconds):
+    """Per second over a pattern's timed match: candidate-vs-list probes
+    (every candidate streamed from a pivot list, times max(1, |B| - 1) lists
+    it is tested against) and set intersections (paths times |B| - 1), over
+    the intersecting levels (COMPUTE, COMPUTE_COUNT, COMPUTE_PATH_COUNT).
+    `ops` are the plan's (op, vertex) levels, `levels` their stats."""
+    back = {}
+    for line in plan_text.splitlines():
+        if line.startswith("backward:"):
+            rows = line[len("backward:"):].strip()[1:-1].split("][") if line.strip() != "backward:" else []
+            back = {u: [x for x in r.split(",") if x] for u, r in enumerate(rows)}
+    probes = sets = 0
+    for (op, u), st in zip(ops, levels):
+        if op in ("COMPUTE", "COMPUTE_COUNT", "COMPUTE_PATH_COUNT"):
+            b = len(back.get(u, []))
+            probes += st["work"] * max(1, b - 1)
+            sets += st["paths"] * max(0, b - 1)
+    if seconds <= 0:
+        return None
+    return {"probes_per_s": probes / seconds, "set_intersections_per_s": sets / seconds}
+
+
 def cpu_baseline(O, rp, cl, plan_texts, budget_s, threads, seed=12345):
     """Oracle (C restatement of the level algorithm, OpenMP over roots) on a
     seeded uniform sample of root vertices, all patterns. The sample grows
@@ -241,6 +263,15 @@ def main():
     for k in keys:
         for lvl, st in enumerate(level_stats[k]):
             log(f"{k} L{lvl} batches={st['batches']} paths={st['paths']} work={st['work']} outputs={st['outputs']}")
+    for k, p, m in zip(keys, plans, match_ms):
+        try:  # reported beside the metric; never allowed to stop the line
+            rates = intersection_rates(p.serialize(), p.ops, [{**st, "work": st["work"] // args.steps,
+                                                               "paths": st["paths"] // args.steps} for st in level_stats[k]],
+                                       m / 1000.0)
+        except Exception as ex_:  # noqa: BLE001
+            log(f"intersection rates of {k} unavailable: {ex_}")
+            rates = None
+        per_pattern[f"{NAMES.get(k, k)}({k})"]["intersections"] = rates
 
     cpu = None
     if not args.no_cpu_baseline:
