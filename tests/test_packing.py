@@ -248,3 +248,18 @@ def test_random_graphs_pack_and_partition_like_the_reference_tools(vsgm, tmp_pat
                      open(base + ".vbstat").read() if r.returncode == 0 else None, f.returncode,
                      open(f"{gf}.fennel.part.{parts}").read()))
     assert outs[0] == outs[1]
+
+
+FEATURES = json.load(open(os.path.join(ROOT, "tests", "golden", "features_vectors.json")))
+
+
+@pytest.mark.parametrize("i", range(len(FEATURES)))
+def test_features_golden_vectors(vsgm, tmp_path, i):
+    import hashlib
+
+    scale, ef, seed, flags, out = FEATURES[i]["case"]
+    gf = str(tmp_path / "g.bin")
+    vsgm.Graph.rmat(scale, ef, seed=seed).write_bin(gf)
+    r = subprocess.run([os.path.join(ROOT, "vsgm_amd", "bin", "features"), "-f", gf] + flags, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert hashlib.sha256(open(f"{gf}.{out}.bin", "rb").read()).hexdigest() == FEATURES[i]["sha256"]
