@@ -243,8 +243,10 @@ int vsgm_clusters_reorder(const vsgm_graph* g, int k, int32_t* assignment);
 int vsgm_clusters_compact(int k, int32_t* assignment, uint64_t vertex_count, int* kept);
 
 /* ---- the preprocess_graph tool (src/preprocess/preprocess_graph.cc:9-36):
- * -f SNAP file, -d directed (0 only), -o 1 (.bin), plus -g GPU (>= 0 builds
- * the CSR there) and -t threads; writes <file>.bin */
+ * -f graph file, -d directed (0 only), -o 0 (.snap.txt), 1 (.bin, the
+ * default) or 3 (.adj.txt + .info), the reference's GraphFormat numbers,
+ * plus -g GPU (>= 0 builds the CSR there) and -t threads; writes
+ * <file> plus that suffix. */
 int vsgm_preprocess_graph(int argc, char** argv);
 
 #ifdef __cplusplus

@@ -183,14 +183,14 @@ def test_preprocess_graph_text_outputs(vsgm, tmp_path):
     path = tmp_path / "g.txt"
     path.write_text("# c\n4 6\n6 8\n8 4\n8 8\n4 6\n10 12\n")
     exe = os.path.join(ROOT, "vsgm_amd", "bin", "preprocess_graph")
-    for fmt in ("0", "2"):
+    for fmt in ("0", "3"):
         out = subprocess.run([exe, "-f", str(path), "-o", fmt], capture_output=True, text=True, timeout=60)
         assert out.returncode == 0, out.stderr
     # ids 4,6,8,10,12 -> shifted 0..8 -> isolated removed: 0,2,4,6,8 -> 0..4
     assert (tmp_path / "g.txt.snap.txt").read_text() == "0 1\n0 2\n1 2\n3 4\n"
     assert (tmp_path / "g.txt.adj.txt").read_text() == "0\t1 2 \n1\t0 2 \n2\t0 1 \n3\t4 \n4\t3 \n"
     assert (tmp_path / "g.txt.adj.txt.info").read_text() == "5 8"
-    assert subprocess.run([exe, "-f", str(path), "-o", "3"], capture_output=True, timeout=60).returncode == 2
+    assert subprocess.run([exe, "-f", str(path), "-o", "2"], capture_output=True, timeout=60).returncode == 2
 
 
 from hypothesis import given, settings, HealthCheck, strategies as st  # noqa: E402
@@ -220,3 +220,30 @@ def test_snap_reader_matches_oracle_on_random_texts(vsgm, tmp_path, lines, threa
     rp, cols = vsgm.Graph.read_snap(str(path), threads=threads).csr()
     want = I.csr_from_pairs(pairs, V, True, True)
     assert np.array_equal(rp, want[0]) and np.array_equal(cols, want[1])
+
+
+REF_PREPROCESS = os.path.join(ROOT, "oracle", "_ref", "preprocess_graph")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PREPROCESS), reason="oracle/_ref/preprocess_graph is built only where the reference tree exists")
+@pytest.mark.parametrize("seed", range(3))
+def test_preprocess_graph_matches_reference_tool(tmp_path, seed):
+    """The reference's own preprocess_graph (built from its sources) and ours
+    write the same .bin, .snap.txt and .adj.txt(.info) bytes from a SNAP text
+    with comments, self loops, duplicate edges and missing ids."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(50, 900))
+    lines = ["# FromNodeId\tToNodeId"] + [f"{a + 7}\t{b + 7}" for a, b in rng.integers(0, n, size=(int(rng.integers(10, 4000)), 2))]
+    text = "\n".join(lines) + "\n"
+    outs = []
+    for name, tool in (("ref", REF_PREPROCESS), ("me", os.path.join(ROOT, "vsgm_amd", "bin", "preprocess_graph"))):
+        d = tmp_path / name
+        d.mkdir()
+        (d / "g.txt").write_text(text)
+        files = []
+        for fmt, suffixes in (("1", [".bin"]), ("0", [".snap.txt"]), ("3", [".adj.txt", ".adj.txt.info"])):
+            r = subprocess.run([tool, "-f", str(d / "g.txt"), "-d", "0", "-o", fmt], capture_output=True, timeout=120)
+            assert r.returncode == 0, (name, fmt)
+            files += [(d / ("g.txt" + suf)).read_bytes() for suf in suffixes]
+        outs.append(files)
+    assert outs[0] == outs[1]

@@ -295,7 +295,7 @@ double Ms(std::chrono::steady_clock::time_point a) {
 int PreprocessGraphMain(int argc, char** argv) {
   if (argc == 1) {
     std::cout << "./preprocess_graph -f GRAPH_FILENAME(SNAP format) -d DIRECTED -o OUTPUT_FORMAT [-g GPU] [-t THREADS]" << std::endl;
-    std::cout << "OUTPUT_FORMAT: 0 for .snap.txt, 1 (Default) for .bin, 2 for .adj.txt; -g >= 0 builds the CSR on that GPU" << std::endl;
+    std::cout << "OUTPUT_FORMAT: 0 for .snap.txt, 1 (Default) for .bin, 3 for .adj.txt; -g >= 0 builds the CSR on that GPU" << std::endl;
     return -1;
   }
   Args a{argc, argv};
@@ -304,8 +304,10 @@ int PreprocessGraphMain(int argc, char** argv) {
   const int format = a.i("-o", 1);
   const int device = a.i("-g", -1);
   const int threads = a.i("-t", 0);
-  if (format < 0 || format > 2) {
-    std::cerr << "unknown output format " << format << " (0 .snap.txt, 1 .bin, 2 .adj.txt)" << std::endl;
+  // GraphFormat numbering (graph_io.h:22): SNAP 0, BIN 1, SNAP_LVID 2 (an
+  // input format only), ADJ 3
+  if (format != 0 && format != 1 && format != 3) {
+    std::cerr << "unknown output format " << format << " (0 .snap.txt, 1 .bin, 3 .adj.txt)" << std::endl;
     return 2;
   }
   if (directed) throw Error("directed edge lists are not supported; pattern matching reads undirected graphs");
