@@ -247,3 +247,23 @@ def test_preprocess_graph_matches_reference_tool(tmp_path, seed):
             files += [(d / ("g.txt" + suf)).read_bytes() for suf in suffixes]
         outs.append(files)
     assert outs[0] == outs[1]
+
+
+def test_preprocess_graph_golden_vectors(tmp_path):
+    """Committed digests of the reference tool's outputs (written by
+    tests/golden/make_preprocess_vectors.py) against ours."""
+    import hashlib
+    import json
+    import sys
+
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_preprocess_vectors import OUTPUTS, snap_text
+
+    exe = os.path.join(ROOT, "vsgm_amd", "bin", "preprocess_graph")
+    for v in json.load(open(os.path.join(ROOT, "tests", "golden", "preprocess_vectors.json"))):
+        path = str(tmp_path / f"g{v['case'][0]}.txt")
+        open(path, "w").write(snap_text(*v["case"]))
+        for fmt, suffix in OUTPUTS:
+            if not os.path.exists(path + suffix):
+                assert subprocess.run([exe, "-f", path, "-d", "0", "-o", fmt], capture_output=True, timeout=120).returncode == 0
+            assert hashlib.sha256(open(path + suffix, "rb").read()).hexdigest() == v["sha256"][suffix], (v["case"], suffix)
