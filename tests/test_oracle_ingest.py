@@ -226,25 +226,28 @@ REF_PREPROCESS = os.path.join(ROOT, "oracle", "_ref", "preprocess_graph")
 
 
 @pytest.mark.skipif(not os.path.exists(REF_PREPROCESS), reason="oracle/_ref/preprocess_graph is built only where the reference tree exists")
-@pytest.mark.parametrize("seed", range(3))
-def test_preprocess_graph_matches_reference_tool(tmp_path, seed):
+@pytest.mark.parametrize("seed,kind", [(0, "txt"), (1, "txt"), (2, "txt"), (3, "lvid"), (4, "lvid")])
+def test_preprocess_graph_matches_reference_tool(tmp_path, seed, kind):
     """The reference's own preprocess_graph (built from its sources) and ours
     write the same .bin, .snap.txt and .adj.txt(.info) bytes from a SNAP text
-    with comments, self loops, duplicate edges and missing ids."""
+    (or a large-id .lvid text) with comments, self loops, duplicate edges and
+    missing ids."""
     rng = np.random.default_rng(seed)
     n = int(rng.integers(50, 900))
-    lines = ["# FromNodeId\tToNodeId"] + [f"{a + 7}\t{b + 7}" for a, b in rng.integers(0, n, size=(int(rng.integers(10, 4000)), 2))]
+    ids = [str(10 ** 20 + 7919 * i) if kind == "lvid" else str(i + 7) for i in range(n)]
+    lines = ["# FromNodeId\tToNodeId"] + [f"{ids[a]}\t{ids[b]}" for a, b in rng.integers(0, n, size=(int(rng.integers(10, 4000)), 2))]
     text = "\n".join(lines) + "\n"
     outs = []
     for name, tool in (("ref", REF_PREPROCESS), ("me", os.path.join(ROOT, "vsgm_amd", "bin", "preprocess_graph"))):
         d = tmp_path / name
         d.mkdir()
-        (d / "g.txt").write_text(text)
+        src = d / f"g.{kind}"
+        src.write_text(text)
         files = []
         for fmt, suffixes in (("1", [".bin"]), ("0", [".snap.txt"]), ("3", [".adj.txt", ".adj.txt.info"])):
-            r = subprocess.run([tool, "-f", str(d / "g.txt"), "-d", "0", "-o", fmt], capture_output=True, timeout=120)
+            r = subprocess.run([tool, "-f", str(src), "-d", "0", "-o", fmt], capture_output=True, timeout=120)
             assert r.returncode == 0, (name, fmt)
-            files += [(d / ("g.txt" + suf)).read_bytes() for suf in suffixes]
+            files += [(d / (src.name + suf)).read_bytes() for suf in suffixes]
         outs.append(files)
     assert outs[0] == outs[1]
 
