@@ -137,7 +137,9 @@ uint64_t ViewBinManager::GetMaxPartitionedSourcesNum() const {
 }
 
 void ViewBinManager::Reorder() {
-  std::stable_sort(pool_.begin(), pool_.end(), [](const auto& a, const auto& b) { return a->GetComputationFactors() > b->GetComputationFactors(); });
+  // std::sort with the reference's comparator (view_bin_manager.h:136-142),
+  // so bins of equal cost keep the reference's order too
+  std::sort(pool_.begin(), pool_.end(), [](const auto& a, const auto& b) { return a->GetComputationFactors() > b->GetComputationFactors(); });
 }
 
 void ViewBinManager::Split(int device_num, int times) {
@@ -162,13 +164,16 @@ void ViewBinManager::Split(int device_num, int times) {
     }
     std::vector<std::pair<uintV, uint64_t>> by_degree;
     for (auto s : pool_[best]->GetSources()) by_degree.emplace_back(s, graph_->Degree(s));
-    std::stable_sort(by_degree.begin(), by_degree.end(), [](const auto& a, const auto& b) { return a.second > b.second; });
+    // std::sort as the reference sorts (view_bin_manager.h:100), so sources of
+    // equal degree split the same way
+    std::sort(by_degree.begin(), by_degree.end(), [](const auto& a, const auto& b) { return a.second > b.second; });
     std::vector<uintV> odd, even;
     for (size_t i = 0; i < by_degree.size(); ++i) (i % 2 ? odd : even).push_back(by_degree[i].first);
-    int removed_id = pool_[best]->GetId();
     pool_.erase(pool_.begin() + best);
+    // ids above the removed position shift down (view_bin_manager.h:111-115
+    // compares ids with the position, as here)
     for (auto& b : pool_)
-      if (b->GetId() > removed_id) b->SetId(b->GetId() - 1);
+      if (b->GetId() > (int)best) b->SetId(b->GetId() - 1);
     int id = (int)pool_.size();
     pool_.emplace_back(new ViewBin(id, graph_, hop_, threads_));
     pool_.back()->SetSources(std::move(odd));
