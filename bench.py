@@ -3,27 +3,40 @@
 
 Workload (default): seeded R-MAT scale 22, edge factor 16,
 a/b/c/d = .57/.19/.19/.05, made undirected, deduplicated, self loops
-removed, as a u32 CSR; one view bin holding every vertex (a single k-means
-cluster) materialised as a 2-hop view with the root-degree filter; patterns
+removed, as a u32 CSR; one k-means partition with k = 1 (every vertex in one
+view bin) materialised as a 2-hop view with the root-degree filter; patterns
 4-clique (-kc 4), diamond (-p 2) and 4-cycle (-p 1), heuristic plans, one
 pattern per launch sequence. A step is one Match of all three patterns over
 the view resident in HBM; `value` is embeddings enumerated per second over
 the timed steps.
 
+Wall-clock budget. The whole run (graph generation, view, CPU baseline,
+warm-up, timed steps) is kept within --budget-s seconds counted from the
+start of the process: --warmup and --steps are upper bounds, every pattern
+gets at least one full timed step, and the line reports the steps actually
+warmed up and timed. Progress goes to stderr as each stage ends.
+
 The JSON line also carries
   roofline      for the kernel with the most device time in the timed
-                region: algorithmic bytes per launch / average launch time
-                (HIP events on the executor's stream), against 8 TB/s;
-                `traffic` from the rocprof PMC summary in profiles/ when present;
+                region: algorithmic bytes per launch (every candidate
+                element streamed once from its pivot list, plus each path's
+                bound vertices, row_ptrs and segment words) / average launch
+                time (HIP events on the executor's stream), against 8 TB/s;
+                `traffic`: rocprof-measured fabric bytes per launch from
+                profiles/, only when that file was measured on this
+                workload, plan, map kernel and kernel source;
   cpu_baseline  the oracle's C restatement of the level algorithm on the
-                host cores, over a bounded sample of root vertices.
+                host cores, over a bounded sample of root vertices; the GPU
+                counts from the same roots are checked equal to it.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
 import time
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -31,15 +44,25 @@ PEAK_HBM_GBS = 8000.0
 PATTERNS = {"kc3": ("clique", 3), "kc4": ("clique", 4), "kc5": ("clique", 5)}
 PATTERNS.update({f"p{k}": ("preset", k) for k in range(19)})
 NAMES = {"kc3": "triangle", "kc4": "4-clique", "kc5": "5-clique", "p1": "4-cycle", "p2": "diamond", "p4": "house"}
+KERNEL_SOURCE = os.path.join(ROOT, "vsgm_amd", "csrc", "hip", "vsgm_hip.hip")
+
+
+def elapsed():
+    return time.perf_counter() - T_START
 
 
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {elapsed():7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def make_query(V, key):
     kind, k = PATTERNS[key]
     return V.Query.clique(k) if kind == "clique" else V.Query.preset(k)
+
+
+def kernel_source_sha():
+    with open(KERNEL_SOURCE, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
 def intersection_rates(plan_text, ops, levels, seconds):
@@ -67,24 +90,26 @@ def intersection_rates(plan_text, ops, levels, seconds):
 def cpu_baseline(O, rp, cl, plan_texts, budget_s, threads, seed=12345):
     """Oracle (C restatement of the level algorithm, OpenMP over roots) on a
     seeded uniform sample of root vertices, all patterns. The sample grows
-    4x until one pass takes at least budget_s / 4 seconds; the rate of the
-    last pass is reported (embeddings found from the sampled roots per
-    second)."""
+    4x while the next pass is predicted to fit budget_s (a pass takes at
+    least budget_s / 8 unless every root is in); the rate of the last pass
+    is reported (embeddings found from the sampled roots per second),
+    together with its roots and per-pattern counts."""
     import numpy as np
 
     n = len(rp) - 1
     rng = np.random.default_rng(seed)
     order = rng.permutation(n).astype(np.uint32)
     m = 64
+    spent = 0.0
     while True:
         roots = np.sort(order[: min(m, n)])
         t = time.perf_counter()
-        total = 0
-        for txt in plan_texts:
-            total += O.count_plan(rp, cl, txt, sources=roots, threads=threads)
+        per = [O.count_plan(rp, cl, txt, sources=roots, threads=threads) for txt in plan_texts]
         dt = time.perf_counter() - t
+        spent += dt
+        total = sum(per)
         log(f"cpu baseline sample {len(roots)} roots: {total} embeddings in {dt:.2f}s")
-        if dt >= budget_s / 4 or m >= n:
+        if dt * 4 > budget_s - spent or dt >= budget_s / 8 or m >= n:
             return {
                 "value": total / dt if dt > 0 else 0.0,
                 "unit": "embeddings/s",
@@ -92,25 +117,31 @@ def cpu_baseline(O, rp, cl, plan_texts, budget_s, threads, seed=12345):
                 "kind": "port",
                 "sample": f"{len(roots)} uniformly drawn roots (seed {seed}) of {n} vertices, all patterns; "
                 f"{total} embeddings in {dt:.2f} s",
-            }
+            }, roots, per
         m *= 4
 
 
-def load_traffic(kernel, workload):
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+def load_traffic(kernel, workload, plan, isect_kernel):
+    """Measured traffic of `kernel` on this exact configuration, or None: the
+    profile file must name the same workload, plan, map kernel variant and
+    kernel source (sha256 prefix of vsgm_hip.hip)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}_{plan}_k{isect_kernel}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        d = json.load(f)
+    if d.get("kernel_source_sha") != kernel_source_sha():
+        log(f"traffic profile {os.path.basename(path)} was measured on another kernel source; not used")
+        return None
+    return d
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # one step of the scale-22 workload takes minutes (the 4-cycle level
-    # visits ~5e13 candidate elements), so the defaults time a single step
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=20, help="upper bound of timed steps (the budget may allow fewer)")
+    ap.add_argument("--warmup", type=int, default=1, help="upper bound of untimed warm-up steps")
+    ap.add_argument("--budget-s", type=float, default=480.0, help="wall-clock budget of the whole run from process start")
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
@@ -120,7 +151,9 @@ def main():
     ap.add_argument("--detail-out", default="")
     ap.add_argument("--hub-bytes", type=float, default=-1, help="membership index bytes (-1 auto, 0 off)")
     ap.add_argument("--hub-min-degree", type=int, default=64)
-    ap.add_argument("--isect-kernel", type=int, default=0)
+    ap.add_argument("--isect-kernel", type=int, default=3)
+    ap.add_argument("--cold-weight", type=int, default=2)
+    ap.add_argument("--ticket-chunks", type=int, default=64)
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
     ap.add_argument("--cl-vertices", type=int, default=65_600_000)
     ap.add_argument("--cl-edges", type=float, default=1.81e9, help="target undirected edges of the Chung-Lu graph")
@@ -148,7 +181,10 @@ def main():
     if args.workload == "chunglu":
         return chunglu_main(args)
 
+    import numpy as np
+
     import vsgm_amd as V
+    from vsgm_amd import partition
 
     assert V.device_count() > 0, "no GPU visible"
     keys = [k.strip() for k in args.patterns.split(",") if k.strip()]
@@ -164,131 +200,121 @@ def main():
     plans = [V.Plan.optimize(q) for q in queries]
     learned_info = None
     if args.plan == "learned":
-        # the learned planner: records from small seeded R-MAT graphs, HIP-trained
-        # cost model, order per pattern (heuristic plan kept where no order is
-        # predicted to gain)
-        import numpy as np
-
-        from vsgm_amd import learned
-
-        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-        from shapes import all_shapes
-
-        t = time.perf_counter()
-        tq = [V.Query.from_matrix(s[2]) for s in all_shapes((3, 4, 5))]
-        data = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (10, 11)], tq, bins=2, max_orders=24, repeats=1,
-                               seed=args.seed)
-        # 4-vertex shapes also on larger graphs, where order costs separate
-        # from launch overheads (scale 11 times are ~1 ms)
-        q4 = [V.Query.from_matrix(s[2]) for s in all_shapes((4,))]
-        big = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (13, 14)], q4, bins=2, max_orders=24, repeats=1,
-                              seed=args.seed + 1)
-        big["group"] = big["group"] + int(data["group"].max()) + 1
-        data = {k: np.concatenate([data[k], big[k]]) for k in data}
-        model, losses = learned.train(data, epochs=60, batch=128, lr=2e-3, seed=args.seed)
-        rp_all, _ = g.csr(copy=False)
-        vstats = learned.view_stats(rp_all, g.vertex_count)
-        planner = learned.LearnedPlanner(model, margin=0.1)
-        chosen = [planner.plan(q, vstats) for q in queries]
-        plans = [p for p, _ in chosen]
-        learned_info = {"records": int(len(data["y"])), "loss": [float(losses[0]), float(losses[-1])],
-                        "orders": {k: (info.get("order") if info["learned"] else "heuristic") for k, (_, info) in zip(keys, chosen)},
-                        "predicted_gain": {k: (round(info["pred_gain"], 4), info.get("best_order", info.get("order")))
-                                           for k, (_, info) in zip(keys, chosen)},
-                        "train_s": round(time.perf_counter() - t, 1)}
-        log(f"learned planner: {learned_info}")
+        plans, learned_info = learned_plans(V, args, keys, queries, g)
     hop = max(p.hop for p in plans)
     root_degree = min(p.root_degree for p in plans)
 
-    # one view bin = all vertices (single cluster), hop-hop view with the root-degree filter
+    # one k-means partition with k = 1 on the GPU (every vertex in cluster 0),
+    # packed as one view bin: the hop-hop view with the root-degree filter
     t = time.perf_counter()
-    vbm = V.ViewBinManager.from_assignment(g, hop, 1, [0] * g.vertex_count)
+    assign, km_sizes, _ = partition.kmeans(g, 1, iters=1, seed=args.seed)
+    vbm = V.ViewBinManager.from_assignment(g, hop, 1, assign)
     holder = V.ViewBinHolder(g.vertex_count, max(4, g.entries * 4))
     vbm.materialize(0, holder, root_degree)
     view_s = time.perf_counter() - t
     info = holder.info()
-    log(f"view: hop={hop} root_degree={root_degree} vertices={info['vertices']} entries={info['entries']} in {view_s:.1f}s")
+    log(f"k-means k=1 ({km_sizes.tolist()}) + view: hop={hop} root_degree={root_degree} vertices={info['vertices']} "
+        f"entries={info['entries']} in {view_s:.1f}s")
 
     ex = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
     ex.set_option("hub_bytes", args.hub_bytes)
     ex.set_option("hub_min_degree", args.hub_min_degree)
     ex.set_option("isect_kernel", args.isect_kernel)
+    ex.set_option("cold_weight", args.cold_weight)
+    ex.set_option("ticket_chunks", args.ticket_chunks)
     t = time.perf_counter()
     ex.transfer(holder)
     xfer_s = time.perf_counter() - t
     log(f"transfer {info['entries'] * 4 / 1e9 + (g.vertex_count + 1) * 8 / 1e9:.2f} GB + membership index of "
         f"{ex.hub_count()} lists in {xfer_s:.2f}s")
 
+    # CPU baseline (and the parity sample) before the GPU steps, so the steps
+    # get whatever budget is left
+    cpu, roots, cpu_counts, sample_check = None, None, None, None
+    rp, cl = g.csr(copy=False)
+    if not args.no_cpu_baseline:
+        import oracle as O
+
+        threads = min(os.cpu_count() or 1, 16)
+        cpu, roots, cpu_counts = cpu_baseline(O, rp, cl, [p.serialize() for p in plans], args.cpu_seconds, threads)
+        log(f"cpu baseline: {cpu}")
+        sx = V.PipelineExecutor(0, g.vertex_count, plans)
+        sx.set_option("isect_kernel", args.isect_kernel)
+        sx.transfer_graph(g, roots)
+        sx.match()
+        sample_check = {"roots": int(len(roots)), "gpu": sx.total_counts, "oracle": cpu_counts,
+                        "equal": sx.total_counts == cpu_counts}
+        del sx
+        log(f"sampled-root parity: gpu {sample_check['gpu']} oracle {cpu_counts}")
+        assert sample_check["equal"], f"GPU counts {sample_check['gpu']} differ from the oracle {cpu_counts} on the sampled roots"
+
+    # warm-up: only while a timed step would still fit after it
+    reserve = 15.0  # line assembly and process exit
+    est = None
+    warm = 0
     for w in range(args.warmup):
+        if est is not None and elapsed() + 2 * est + reserve > args.budget_s:
+            break
         t = time.perf_counter()
         ex.match()
-        log(f"warmup {w}: {time.perf_counter() - t:.2f}s counts={ex.total_counts}")
-    ref_counts = [c // max(args.warmup, 1) for c in ex.total_counts] if args.warmup else None
+        est = time.perf_counter() - t
+        warm += 1
+        log(f"warmup {w}: {est:.2f}s counts={ex.total_counts}")
+    if est is not None and elapsed() + est + reserve > args.budget_s:
+        log(f"warm-up step took {est:.1f}s; one timed step still runs (every pattern gets one)")
+    ref_counts = [c // warm for c in ex.total_counts] if warm else None
 
     ex.reset()
     ex.profile(True)
     t0 = time.perf_counter()
-    for s in range(args.steps):
+    steps = 0
+    while steps < max(args.steps, 1):
+        if steps >= 1 and elapsed() + est + reserve > args.budget_s:
+            break
         ts = time.perf_counter()
         ex.match()
-        log(f"step {s}: {time.perf_counter() - ts:.2f}s")
+        est = time.perf_counter() - ts
+        steps += 1
+        log(f"step {steps - 1}: {est:.2f}s")
     t1 = time.perf_counter()
     kstats = ex.kernel_stats()
     ex.profile(False)
 
-    counts = [c // args.steps for c in ex.total_counts]
-    assert all(c * args.steps == tc for c, tc in zip(counts, ex.total_counts)), "counts differ between steps"
+    counts = [c // steps for c in ex.total_counts]
+    assert all(c * steps == tc for c, tc in zip(counts, ex.total_counts)), "counts differ between steps"
     if ref_counts is not None:
         assert counts == ref_counts, f"timed counts {counts} differ from warmup counts {ref_counts}"
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    match_ms = [m / args.steps for m in ex.match_ms]
+    ms_per_step = (t1 - t0) * 1000.0 / steps
+    match_ms = [m / steps for m in ex.match_ms]
     value = sum(counts) / (ms_per_step / 1000.0)
     per_pattern = {f"{NAMES.get(k, k)}({k})": {"count": c, "ms": round(m, 3), "embeddings_per_s": c / (m / 1000.0) if m else 0.0}
                    for k, c, m in zip(keys, counts, match_ms)}
 
-    dom = max(kstats, key=lambda s: s["total_ms"]) if kstats else None
-    roofline = None
-    if dom:
-        per_launch_bytes = dom["algo_bytes"] / dom["launches"]
-        per_launch_s = dom["total_ms"] / dom["launches"] / 1000.0
-        achieved = per_launch_bytes / per_launch_s / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(dom["name"], workload),
-                    "kernel": dom["name"], "launches": dom["launches"], "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 3),
-                    "algo_bytes_per_launch": int(per_launch_bytes)}
-    for s in sorted(kstats, key=lambda s: -s["total_ms"]):
-        log(f"kernel {s['name']:16s} launches={s['launches']:6d} total={s['total_ms']:10.2f}ms "
-            f"algoGB/s={s['algo_bytes'] / max(s['total_ms'], 1e-9) / 1e6:8.1f} work={s['work']:.3e} paths={s['paths']:.3e}")
+    roofline = make_roofline(kstats, workload, args.plan, args.isect_kernel)
+    for s_ in sorted(kstats, key=lambda s: -s["total_ms"]):
+        log(f"kernel {s_['name']:16s} launches={s_['launches']:6d} total={s_['total_ms']:10.2f}ms "
+            f"algoGB/s={s_['algo_bytes'] / max(s_['total_ms'], 1e-9) / 1e6:8.1f} work={s_['work']:.3e} paths={s_['paths']:.3e}")
     level_stats = {k: ex.level_stats(i) for i, k in enumerate(keys)}
     for k in keys:
         for lvl, st in enumerate(level_stats[k]):
             log(f"{k} L{lvl} batches={st['batches']} paths={st['paths']} work={st['work']} outputs={st['outputs']}")
     for k, p, m in zip(keys, plans, match_ms):
         try:  # reported beside the metric; never allowed to stop the line
-            rates = intersection_rates(p.serialize(), p.ops, [{**st, "work": st["work"] // args.steps,
-                                                               "paths": st["paths"] // args.steps} for st in level_stats[k]],
+            rates = intersection_rates(p.serialize(), p.ops, [{**st, "work": st["work"] // steps,
+                                                               "paths": st["paths"] // steps} for st in level_stats[k]],
                                        m / 1000.0)
         except Exception as ex_:  # noqa: BLE001
             log(f"intersection rates of {k} unavailable: {ex_}")
             rates = None
         per_pattern[f"{NAMES.get(k, k)}({k})"]["intersections"] = rates
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        import oracle as O
-
-        rp, cl = g.csr(copy=False)
-        threads = min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(O, rp, cl, [p.serialize() for p in plans], args.cpu_seconds, threads)
-        log(f"cpu baseline: {cpu}")
-
     line = {
         "metric": "embeddings/sec per pattern, 1 MI355X; learned vs heuristic plan, exact counts",
         "value": value,
         "unit": "embeddings/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warm,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -296,8 +322,8 @@ def main():
         "dtype": "u32",
         "data": f"synthetic: seeded R-MAT (seed {args.seed}) generated on the host",
         "config": {
-            "workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (.57/.19/.19/.05), single-bin {hop}-hop view, "
-                        f"patterns {','.join(keys)}, {args.plan} plans",
+            "workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (.57/.19/.19/.05), k-means k=1 single-bin "
+                        f"{hop}-hop view, patterns {','.join(keys)}, {args.plan} plans",
             "scale": args.scale,
             "vertices": g.vertex_count,
             "edges": g.edge_count,
@@ -306,11 +332,18 @@ def main():
             "per_pattern": per_pattern,
             "plan": args.plan,
             "learned_planner": learned_info,
+            "steps_requested": args.steps,
+            "warmup_requested": args.warmup,
+            "budget_s": args.budget_s,
+            "wall_s_at_line": round(elapsed(), 1),
             "graph_gen_s": round(gen_s, 2),
             "view_build_s": round(view_s, 2),
             "transfer_s": round(xfer_s, 3),
             "index_lists": ex.hub_count(),
             "isect_kernel": args.isect_kernel,
+            "cold_weight": args.cold_weight,
+            "ticket_chunks": args.ticket_chunks,
+            "sampled_root_parity": sample_check,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -319,6 +352,74 @@ def main():
         with open(args.detail_out, "w") as f:
             json.dump({"line": line, "kernels": kstats, "levels": level_stats}, f, indent=1)
     print(json.dumps(line), flush=True)
+
+
+def make_roofline(kstats, workload, plan, isect_kernel):
+    """Roofline of the kernel with the most device time: algorithmic bytes
+    per launch over the average launch time against the HBM peak. When the
+    algorithmic rate exceeds the peak (candidate lists re-read from the
+    caches), `frac` is the measured fabric traffic rate over the peak
+    instead, and `bound_note` says so; with no matching traffic profile it
+    is capped at 1 and labelled as cache-served."""
+    if not kstats:
+        return None
+    dom = max(kstats, key=lambda s: s["total_ms"])
+    per_launch_bytes = dom["algo_bytes"] / dom["launches"]
+    per_launch_s = dom["total_ms"] / dom["launches"] / 1000.0
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    tr = load_traffic(dom["name"], workload, plan, isect_kernel)
+    traffic = tr["bytes_per_launch"] if tr else None
+    frac = achieved / PEAK_HBM_GBS
+    note = "algorithmic bytes (each pivot candidate streamed once + per-path words) / launch time"
+    if frac > 1.0:
+        if traffic:
+            frac = traffic / per_launch_s / 1e9 / PEAK_HBM_GBS
+            note = "algorithmic rate exceeds HBM peak (lists re-read from cache): frac = measured fabric bytes / launch time / peak"
+        else:
+            frac = 1.0
+            note = "algorithmic rate exceeds HBM peak (lists re-read from cache); no matching traffic profile: frac capped at 1"
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": round(min(frac, 1.0), 4), "traffic": int(traffic) if traffic else None,
+           "kernel": dom["name"], "launches": dom["launches"], "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 3),
+           "algo_bytes_per_launch": int(per_launch_bytes), "bound_note": note}
+    if traffic:
+        out["traffic_gbs"] = round(traffic / per_launch_s / 1e9, 1)
+        out["traffic_profile"] = f"profiles/traffic_{dom['name']}_{workload}_{plan}_k{isect_kernel}.json"
+    return out
+
+
+def learned_plans(V, args, keys, queries, g):
+    """The learned planner on this workload: records from small seeded R-MAT
+    graphs, a HIP-trained cost model, one order per pattern (the heuristic
+    plan kept where no order is predicted to gain)."""
+    import numpy as np
+
+    from vsgm_amd import learned
+    from vsgm_amd.shapes import all_shapes
+
+    t = time.perf_counter()
+    tq = [V.Query.from_matrix(s[2]) for s in all_shapes((3, 4, 5))]
+    data = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (10, 11)], tq, bins=2, max_orders=24, repeats=1,
+                           seed=args.seed)
+    # 4-vertex shapes also on larger graphs, where order costs separate
+    # from launch overheads (scale 11 times are ~1 ms)
+    q4 = [V.Query.from_matrix(s[2]) for s in all_shapes((4,))]
+    big = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (13, 14)], q4, bins=2, max_orders=24, repeats=1,
+                          seed=args.seed + 1)
+    big["group"] = big["group"] + int(data["group"].max()) + 1
+    data = {k: np.concatenate([data[k], big[k]]) for k in data}
+    model, losses = learned.train(data, epochs=60, batch=128, lr=2e-3, seed=args.seed)
+    rp_all, _ = g.csr(copy=False)
+    vstats = learned.view_stats(rp_all, g.vertex_count)
+    planner = learned.LearnedPlanner(model, margin=0.1)
+    chosen = [planner.plan(q, vstats) for q in queries]
+    info = {"records": int(len(data["y"])), "loss": [float(losses[0]), float(losses[-1])],
+            "orders": {k: (c.get("order") if c["learned"] else "heuristic") for k, (_, c) in zip(keys, chosen)},
+            "predicted_gain": {k: (round(c["pred_gain"], 4), c.get("best_order", c.get("order")))
+                               for k, (_, c) in zip(keys, chosen)},
+            "train_s": round(time.perf_counter() - t, 1)}
+    log(f"learned planner: {info}")
+    return [p for p, _ in chosen], info
 
 
 def chunglu_main(args):
@@ -423,9 +524,7 @@ def batched_main(args):
 
     import vsgm_amd as V
     from vsgm_amd import autotune, learned
-
-    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-    from shapes import all_shapes
+    from vsgm_amd.shapes import all_shapes
 
     assert V.device_count() > 0, "no GPU visible"
     rng = np.random.default_rng(args.seed)
@@ -508,7 +607,8 @@ def batched_main(args):
         tx.set_graph(g)
         tx.add_device_view(bin_sources[0], v_hop, v_rd)
         t = time.perf_counter()
-        res = autotune.tune(tx, repeats=1, candidates={"isect_kernel": [0, 2], "max_grid": [8192], "batch_bytes": [0, 1 << 30]})
+        res = autotune.tune(tx, repeats=1, candidates={"isect_kernel": [3, 0], "max_grid": [8192], "batch_bytes": [0, 1 << 30],
+                                                       "ticket_chunks": [16, 64]})
         tune_s = time.perf_counter() - t
         del tx
         bx = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
@@ -528,7 +628,7 @@ def batched_main(args):
         bx.replay()
         run_s = time.perf_counter() - t
         counts = bx.total_counts
-        log(f"{label}: tuned in {tune_s:.1f}s ({sum(r['config'] != {'isect_kernel': 0, 'max_grid': 8192, 'batch_bytes': 0} for r in res.values())} "
+        log(f"{label}: tuned in {tune_s:.1f}s ({sum(r['config'] != {'isect_kernel': 3, 'max_grid': 8192, 'batch_bytes': 0, 'cold_weight': 2, 'ticket_chunks': 64} for r in res.values())} "
             f"plans off the default), captured {nodes} nodes in {cap_s:.1f}s, replay {run_s:.3f}s")
         return counts, run_s, nodes
 

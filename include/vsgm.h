@@ -130,14 +130,17 @@ int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* con
  * dense membership index of the highest-degree view lists, built at the next
  * transfer; 0 = off, -1 = min(budget/8, 16 GiB), the default),
  * "hub_min_degree" (smallest indexed degree, default 64), and the device
- * tuning knobs "isect_kernel" and "max_grid" (vsgm_tuning in vsgm_hip.h). */
+ * tuning knobs "isect_kernel", "max_grid", "cold_weight" and "ticket_chunks"
+ * (vsgm_tuning in vsgm_hip.h), applied to every plan. */
 int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value);
 /* Number of view vertices in the current membership index. */
 int vsgm_executor_hub_count(const vsgm_executor* e, uint32_t* n);
 /* Launch shape and batch budget of one plan (0 = the executor option),
  * as chosen by the autotuner (vsgm_amd/autotune.py). */
-int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_kernel, uint32_t max_grid, uint64_t batch_bytes);
-int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* isect_kernel, uint32_t* max_grid, uint64_t* batch_bytes);
+int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_kernel, uint32_t max_grid, uint64_t batch_bytes,
+                                  uint32_t cold_weight, uint32_t ticket_chunks);
+int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* isect_kernel, uint32_t* max_grid, uint64_t* batch_bytes,
+                                  uint32_t* cold_weight, uint32_t* ticket_chunks);
 
 /* ---- batched mode (no counterpart in the reference, which uploads one
  * view bin at a time and runs plans one after another with a host

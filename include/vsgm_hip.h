@@ -236,10 +236,24 @@ typedef struct vsgm_kernel_stat {
  *                 2: as 0, and each workgroup keeps in LDS one backward
  *                    adjacency list (<= 4096 entries) bound to the same vertex
  *                    across its chunk and searches it there;
- *   max_grid      cap on workgroups per map launch (grid-stride beyond). */
+ *                 3: COMPUTE_COUNT / per-path counts with two or more
+ *                    backward lists: one 1024-thread workgroup per CU keeps
+ *                    the longest list (<= 14336 entries) whose vertex stays
+ *                    bound across a run of consecutive paths in an LDS hash
+ *                    table, rebuilt when the run changes, and takes tickets
+ *                    of ticket_chunks consecutive 8192-element chunks from a
+ *                    device counter; the pivot choice then prefers streaming
+ *                    the list that changes per path (other ops run as 0);
+ *   max_grid      cap on workgroups per map launch (grid-stride beyond);
+ *   cold_weight   pivot cost of one test against a list that changes per
+ *                 path, in units of one streamed pivot element (isect_kernel
+ *                 3; 1..1024);
+ *   ticket_chunks chunks per ticket of isect_kernel 3 (1..64). */
 typedef struct vsgm_tuning {
   uint32_t isect_kernel;
   uint32_t max_grid;
+  uint32_t cold_weight;
+  uint32_t ticket_chunks;
 } vsgm_tuning;
 int vsgm_set_tuning(vsgm_ctx* ctx, const vsgm_tuning* t);
 int vsgm_get_tuning(const vsgm_ctx* ctx, vsgm_tuning* t);

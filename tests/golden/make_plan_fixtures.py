@@ -3,7 +3,7 @@
 Runs oracle/_ref/ref_plan_dump (built by oracle/Makefile from the
 reference's own src/query headers and vendored bliss sources) on:
   preset 0..18, clique 3..7, graphlets 3..5,
-  and every connected shape on 3..6 vertices (tests/golden/shapes.py).
+  and every connected shape on 3..6 vertices (vsgm_amd/shapes.py).
 Only the resulting text (inputs and the reference's plans) is committed.
 Run in the container that has the reference tree:
     make -C oracle && python tests/golden/make_plan_fixtures.py
@@ -14,9 +14,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
-sys.path.insert(0, HERE)
+sys.path.insert(0, ROOT)
 
-from shapes import all_shapes  # noqa: E402
+from vsgm_amd.shapes import all_shapes  # noqa: E402
 
 
 def specs():

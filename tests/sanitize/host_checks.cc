@@ -183,6 +183,16 @@ void CApi(const std::string& dir) {
   Expect(vsgm_graph_read_bin(nullptr, &r) != 0 && vsgm_graph_read_bin((dir + "/c.bin").c_str(), nullptr) != 0, "null path / out reported");
   Expect(vsgm_plan_optimize(nullptr, &p) != 0 && vsgm_query_preset(1, nullptr) != 0, "null query / out reported");
   Expect(vsgm_executor_match(nullptr, 1) != 0 && vsgm_executor_counts(nullptr, nullptr, 0) != 0, "null executor reported");
+  // executor arguments are checked before the device is looked for
+  const vsgm_plan* null_plans[1] = {nullptr};
+  Expect(vsgm_executor_create(0, 512, nullptr, 1, 512, 0, &e) == VSGM_ERR_ARG && e == nullptr, "null plan list reported");
+  Expect(vsgm_executor_create(0, 512, null_plans, 1, 512, 0, &e) == VSGM_ERR_ARG && e == nullptr, "null plan reported");
+  Expect(vsgm_executor_create(0, 512, plans, 1, 512, 0, nullptr) == VSGM_ERR_ARG, "null executor out reported");
+  Expect(vsgm_executor_create(0, 512, plans, -1, 512, 0, &e) == VSGM_ERR_ARG && e == nullptr, "negative plan count reported");
+  uint32_t u32 = 0;
+  uint64_t u64v = 0;
+  Expect(vsgm_executor_hub_count(nullptr, &u32) != 0 && vsgm_executor_peak_bytes(nullptr, &u64v) != 0, "null executor accessors reported");
+  Expect(vsgm_executor_get_plan_tuning(nullptr, 0, &u32, &u32, &u64v, &u32, &u32) != 0, "null executor tuning reported");
   int n_graphlets = 0;
   Expect(vsgm_query_graphlets(4, nullptr, 0, &n_graphlets) == 0 && n_graphlets == 6, "graphlet count without storage");
   vsgm_plan_free(p);

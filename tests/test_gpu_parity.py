@@ -10,7 +10,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from shapes import all_shapes
+from vsgm_amd.shapes import all_shapes
 
 pytestmark = pytest.mark.gpu
 
@@ -94,7 +94,8 @@ def test_hub_index_invariance(vsgm, oracle, rmat9):
     largest lists only, and on for every non-empty list."""
     plans = [vsgm.Plan.optimize(q) for q in [vsgm.Query.preset(k) for k in range(19)] + [vsgm.Query.clique(5)]]
     want = oracle_counts(oracle, rmat9, plans)
-    for opts in ({"hub_bytes": 0}, {"hub_min_degree": 32}, {"hub_min_degree": 1}, {"hub_min_degree": 1, "hub_bytes": 64 * 40}):
+    for opts in ({"hub_bytes": 0}, {"hub_min_degree": 32}, {"hub_min_degree": 1}, {"hub_min_degree": 1, "hub_bytes": 64 * 40},
+                 {"isect_kernel": 0}, {"isect_kernel": 0, "hub_bytes": 0}):
         print(f"[gpu] hub index {opts}", flush=True)
         assert run_gpu(vsgm, rmat9, plans, **opts) == want, opts
     ex = vsgm.PipelineExecutor(0, rmat9.vertex_count, plans[:1])
@@ -165,6 +166,20 @@ def test_edge_cases(vsgm, oracle, gpu):
     g = vsgm.Graph.rmat(8, 16, seed=1)
     src = list(range(0, g.vertex_count, 3))
     assert run_gpu(vsgm, g, [tri, sq], sources=src) == oracle_counts(oracle, g, [tri, sq], sources=src)
+    # a source outside the vertex range is refused on the host, before any upload or launch,
+    # on every route a view takes (whole graph, device-built, host holder)
+    ex = vsgm.PipelineExecutor(0, g.vertex_count, [tri])
+    bad = [0, 5, g.vertex_count]
+    with pytest.raises(vsgm.VsgmError, match="is not a vertex"):
+        ex.transfer_graph(g, bad)
+    ex.set_graph(g)
+    with pytest.raises(vsgm.VsgmError, match="is not a vertex"):
+        ex.transfer_device(bad, 2, 0)
+    with pytest.raises(vsgm.VsgmError, match="is not a vertex"):
+        ex.add_device_view(bad, 2, 0)
+    ex.transfer_graph(g, src)  # the executor still works afterwards
+    ex.match()
+    assert ex.total_counts == oracle_counts(oracle, g, [tri], sources=src)
 
 
 def test_large_hub_graph(vsgm, oracle, gpu):
@@ -289,8 +304,10 @@ def test_kmeans_tool_reads_features_graphs_as_the_reference_does(vsgm, gpu, tmp_
 
 
 def test_pm_pipeline_checkpoint_resume(vsgm, oracle, gpu, tmp_path):
-    """-ck FILE: each finished bin appends its counts; a rerun skips the
-    bins listed (if their sources still match) and totals stay exact."""
+    """-ck FILE: each finished bin appends its counts under a header keyed
+    to the run (plans, graph, hop, root-degree filter); a rerun skips the
+    bins listed (if their sources still match) and totals stay exact; a run
+    with another query, or a file without the header, is refused."""
     g = vsgm.Graph.rmat(10, 16, seed=14)
     path = str(tmp_path / "g.bin")
     g.write_bin(path)
@@ -299,22 +316,33 @@ def test_pm_pipeline_checkpoint_resume(vsgm, oracle, gpu, tmp_path):
     ck = str(tmp_path / "ck.txt")
     want = oracle_counts(oracle, g, [vsgm.Plan.optimize(vsgm.Query.clique(4))])[0]
 
-    def run(*extra):
-        print("[gpu] pm_pipeline -ck", *extra, flush=True)
-        out = subprocess.run([exe, "-f", path, "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4", "-kc", "4", "-ck", ck] + list(extra),
+    def run(*extra, query=("-kc", "4"), ok=True):
+        print("[gpu] pm_pipeline -ck", *query, *extra, flush=True)
+        out = subprocess.run([exe, "-f", path, "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4", *query, "-ck", ck] + list(extra),
                              capture_output=True, text=True, timeout=120)
+        if not ok:
+            assert out.returncode != 0
+            return None, out.stdout + out.stderr
         assert out.returncode == 0, out.stderr[-2000:]
         line = [x for x in out.stdout.splitlines() if x.startswith("total count for query 0:")][0]
         return int(line.split(":")[1]), out.stdout
 
     total, _ = run()
-    assert total == want and len(open(ck).read().splitlines()) == 4
+    lines = open(ck).read().splitlines()
+    assert total == want and len(lines) == 5 and lines[0].startswith("vsgm-checkpoint ")
     total, out = run()
     assert total == want and "4 of 4 bins already counted" in out
+    # the same file under another query (5-cliques) is refused, untouched
+    _, msg = run(query=("-kc", "5"), ok=False)
+    assert "written for another run" in msg and open(ck).read().splitlines() == lines
     lines = open(ck).read().splitlines()
-    bad = lines[2].split()
+    bad = lines[3].split()
     bad[1] = str(int(bad[1]) ^ 1)  # a bin whose sources no longer match is counted again
-    open(ck, "w").write("\n".join(lines[:2] + [" ".join(bad)]) + "\n")
+    open(ck, "w").write("\n".join(lines[:3] + [" ".join(bad)]) + "\n")
     total, out = run("-dv", "1")
     assert total == want and "2 of 4 bins already counted" in out
-    assert len(open(ck).read().splitlines()) == 5
+    assert len(open(ck).read().splitlines()) == 6
+    # a file without the header line (records of unknown origin) is refused
+    open(ck, "w").write("\n".join(lines[1:]) + "\n")
+    _, msg = run(ok=False)
+    assert "no vsgm-checkpoint header" in msg

@@ -13,7 +13,7 @@ import math
 import numpy as np
 import pytest
 
-from shapes import all_shapes
+from vsgm_amd.shapes import all_shapes
 
 
 def csr(n, edges):

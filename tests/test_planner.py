@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN
-from shapes import all_shapes
+from vsgm_amd.shapes import all_shapes
 
 
 def _records():

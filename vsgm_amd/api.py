@@ -129,8 +129,9 @@ def _declare(L):
         "vsgm_executor_level_stats": ([P, C.c_int, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_peak_bytes": ([P, C.POINTER(u64)], C.c_int),
         "vsgm_executor_hub_count": ([P, C.POINTER(C.c_uint32)], C.c_int),
-        "vsgm_executor_set_plan_tuning": ([P, C.c_int, C.c_uint32, C.c_uint32, u64], C.c_int),
-        "vsgm_executor_get_plan_tuning": ([P, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(u64)], C.c_int),
+        "vsgm_executor_set_plan_tuning": ([P, C.c_int, C.c_uint32, C.c_uint32, u64, C.c_uint32, C.c_uint32], C.c_int),
+        "vsgm_executor_get_plan_tuning": ([P, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(u64),
+                                           C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], C.c_int),
         "vsgm_executor_add_view": ([P, P, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_set_graph": ([P, P], C.c_int),
         "vsgm_executor_add_device_view": ([P, P, u64, C.c_int, u64, C.POINTER(C.c_int)], C.c_int),
@@ -536,13 +537,16 @@ class PipelineExecutor(_Handle):
         _check(lib().vsgm_executor_peak_bytes(self._h, C.byref(b)), "peak")
         return b.value
 
-    def set_plan_tuning(self, plan: int, isect_kernel: int = 0, max_grid: int = 8192, batch_bytes: int = 0):
-        _check(lib().vsgm_executor_set_plan_tuning(self._h, plan, isect_kernel, max_grid, batch_bytes), "plan tuning")
+    def set_plan_tuning(self, plan: int, isect_kernel: int = 3, max_grid: int = 8192, batch_bytes: int = 0, cold_weight: int = 2,
+                        ticket_chunks: int = 64):
+        _check(lib().vsgm_executor_set_plan_tuning(self._h, plan, isect_kernel, max_grid, batch_bytes, cold_weight, ticket_chunks),
+               "plan tuning")
 
     def plan_tuning(self, plan: int) -> dict:
-        k, g, b = C.c_uint32(), C.c_uint32(), u64()
-        _check(lib().vsgm_executor_get_plan_tuning(self._h, plan, C.byref(k), C.byref(g), C.byref(b)), "plan tuning")
-        return {"isect_kernel": k.value, "max_grid": g.value, "batch_bytes": b.value}
+        k, g, b, w, t = C.c_uint32(), C.c_uint32(), u64(), C.c_uint32(), C.c_uint32()
+        _check(lib().vsgm_executor_get_plan_tuning(self._h, plan, C.byref(k), C.byref(g), C.byref(b), C.byref(w), C.byref(t)),
+               "plan tuning")
+        return {"isect_kernel": k.value, "max_grid": g.value, "batch_bytes": b.value, "cold_weight": w.value, "ticket_chunks": t.value}
 
     def add_view(self, holder: ViewBinHolder) -> int:
         """Keep one more view bin resident (batched mode)."""

@@ -48,6 +48,14 @@ constexpr int kPhaseItems = 4;            // elements a thread verifies in one p
 constexpr uint64_t kBeginMask = (1ull << 56) - 1;
 constexpr int kMaxGrid = 8192;
 
+// LDS membership table of the isect_kernel 3 map (one workgroup per CU):
+// two cuckoo tables of kCuckooSlots u32 = 112 KiB; lists of up to kHashCap
+// entries keep the load factor <= 0.45.
+constexpr uint32_t kCuckooSlots = 14336;
+constexpr uint32_t kHashCap = 12800;
+constexpr uint32_t kDefaultColdWeight = 2;
+constexpr uint32_t kDefaultTicketChunks = 64;
+
 enum OpKind { OP_ICOUNT = 0, OP_IWRITE = 1, OP_EXPAND = 2, OP_FILTER = 3, OP_COUNT2 = 4, OP_IPATH = 5 };
 
 __host__ __device__ constexpr bool is_intersect(int op) { return op == OP_ICOUNT || op == OP_IWRITE || op == OP_IPATH; }
@@ -441,28 +449,48 @@ __device__ inline bool stable_binding(const vsgm_level_args& a, uint32_t i, uint
 // cheap to test against (weight 1), one that changes per path costs
 // kColdTest. This streams the fresh list and probes the shared one when the
 // fresh list is at most a few times longer.
+//
+// With hash_cap > 0 (the map kernel keeps one stable list of up to hash_cap
+// entries in an LDS table) testing against that list is nearly free: the
+// stable list with the most entries that fit gets weight 0, and a list that
+// changes per path costs `cold` (one fetched line per test, against 4 bytes
+// per streamed pivot element).
 template <int OP>
 __global__ __launch_bounds__(kBlock) void k_setup(vsgm_level_args a, uint64_t* __restrict__ seg_begin, uint64_t* __restrict__ seg_len,
-                                                  unsigned long long* __restrict__ adj_sum) {
+                                                  unsigned long long* __restrict__ adj_sum, uint32_t hash_cap, uint32_t cold) {
   __shared__ uint64_t s_tmp[kBlock / kWave];
   const uint64_t P = a.path_num;
+  const uint64_t cold_w = hash_cap ? cold : kColdTest;
   uint64_t deg_sum = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < P; p += (uint64_t)gridDim.x * kBlock) {
     uint64_t lo = 0, hi = 0, tag = 0;
     if (is_intersect(OP)) {
+      uint64_t w[VSGM_MAX_QV];
       uint64_t w_total = 0;
-      if (a.n_back > 1)
+      if (a.n_back > 1) {
+        uint32_t hashed = ~0u;
+        uint64_t hashed_n = 0;
         for (uint32_t i = 0; i < a.n_back; ++i) {
           const uint32_t v = a.inst[a.back[i]][p];
-          w_total += stable_binding(a, i, p, v) ? 1 : kColdTest;
+          const bool stable = stable_binding(a, i, p, v);
+          w[i] = stable ? 1 : cold_w;
+          const uint64_t n = a.row_ptrs[v + 1] - a.row_ptrs[v];
+          if (hash_cap && stable && n <= hash_cap && (hashed == ~0u || n > hashed_n)) {
+            hashed = i;
+            hashed_n = n;
+          }
         }
+        if (hashed != ~0u) w[hashed] = 0;
+        for (uint32_t i = 0; i < a.n_back; ++i) w_total += w[i];
+      } else {
+        w[0] = 0;
+      }
       uint64_t best_cost = 0;
       for (uint32_t i = 0; i < a.n_back; ++i) {
         const uint32_t v = a.inst[a.back[i]][p];
         const uint64_t b = a.row_ptrs[v], e = a.row_ptrs[v + 1];
         deg_sum += e - b;
-        const uint64_t w = a.n_back > 1 ? (stable_binding(a, i, p, v) ? 1 : kColdTest) : 0;
-        const uint64_t cost = (e - b) * (1 + w_total - w);
+        const uint64_t cost = (e - b) * (1 + w_total - w[i]);
         if (i == 0 || cost < best_cost) {  // first minimum wins
           best_cost = cost;
           lo = b;
@@ -861,6 +889,36 @@ __device__ inline uint64_t chunk_path(uint64_t e, uint64_t e0, const ChunkMap& c
   return cm.first + i;
 }
 
+// LDS membership table of one adjacency list (the map kernel of
+// isect_kernel 3 keeps the list of a vertex bound across a run of paths
+// here): two-choice cuckoo hashing over two tables of m u32 slots, a key
+// sitting in t0[h0(key)] or t1[h1(key)], so a lookup is two independent LDS
+// reads with no probe loop.
+constexpr uint32_t kEmptySlot = ~0u;
+constexpr int kCuckooRounds = 64;  // evictions one insertion may cause before the build gives up
+
+struct HashView {
+  const uint32_t* t0;
+  const uint32_t* t1;
+  uint32_t m;  // slots in use per table
+  uint32_t v;  // vertex whose list the tables hold (~0u: none)
+};
+
+__device__ inline uint32_t cuckoo_h0(uint32_t c, uint32_t m) { return (uint32_t)(((uint64_t)(c * 0x9E3779B1u) * m) >> 32); }
+
+__device__ inline uint32_t cuckoo_h1(uint32_t c, uint32_t m) {
+  uint32_t x = c ^ (c >> 16);
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  return (uint32_t)(((uint64_t)(x * 0xC2B2AE35u) * m) >> 32);
+}
+
+__device__ inline bool hash_contains(const HashView& h, uint32_t c) {
+  const uint32_t a = h.t0[cuckoo_h0(c, h.m)];
+  const uint32_t b = h.t1[cuckoo_h1(c, h.m)];
+  return a == c || b == c;
+}
+
 // Intersection verification of a lane's elements base + it * 64 + lane of
 // the wavefront's run (it in [it0, it0 + kN)), phase by phase: every phase
 // issues its kN independent loads back to back before any result is used,
@@ -1257,6 +1315,453 @@ __global__ __launch_bounds__(kBlock) void k_map_isect(vsgm_level_args a, const u
   }
 }
 
+// ---------------------------------------------------------------------------
+// Intersection map with the stable list in LDS (isect_kernel 3).
+//
+// Paths arrive in materialization order, so a backward vertex bound before
+// the last one stays the same over long runs of consecutive paths (the
+// 4-cycle's last level: v1 is fixed while v2 walks v0's list). Probing that
+// list through the membership index costs one fetched line per test. Here
+// one 1024-thread workgroup per CU keeps that list in an LDS hash table and
+// rebuilds it only when the run's vertex changes; candidates are still
+// streamed coalesced from the pivot lists. Workgroups take tickets of
+// consecutive chunks from a device counter (dynamic balance, and a ticket's
+// chunks mostly continue one run). A chunk is 8192 elements, 512 per
+// wavefront as in k_map.
+
+constexpr int kHThreads = 1024;
+constexpr int kHWaves = kHThreads / kWave;
+constexpr uint64_t kHChunk = (uint64_t)kHThreads * kItems;
+constexpr int kTicketCap = 64;  // chunks per ticket (upper bound of the tuning knob)
+
+// Per chunk of the launch window: its first and last path, and the vertex
+// whose list the map kernel should hold in LDS for it (~0u: none) with that
+// list's length. The hashed list is the longest backward list of at most
+// kHashCap entries that is bound to the same vertex at both ends of the chunk
+// and is not streamed as the pivot at both ends.
+struct ChunkPlan {
+  uint64_t first;
+  uint64_t last;
+  uint32_t want_v;
+  uint32_t want_n;
+};
+
+__global__ __launch_bounds__(kBlock) void k_chunk_plan(vsgm_level_args a, const uint64_t* __restrict__ seg_begin,
+                                                       const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t c_lo, uint64_t c_hi,
+                                                       ChunkPlan* __restrict__ plan) {
+  const uint64_t P = a.path_num;
+  for (uint64_t k = c_lo + (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < c_hi; k += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t e0 = k * kHChunk, e1 = min(e0 + kHChunk, W);
+    ChunkPlan cp;
+    cp.first = last_le_u64(seg_off, P + 1, e0);
+    cp.last = last_le_u64(seg_off, P + 1, e1 - 1);
+    cp.want_v = ~0u;
+    cp.want_n = 0;
+    if (a.n_back > 1) {
+      const uint32_t t0 = (uint32_t)(seg_begin[cp.first] >> 56), t1 = (uint32_t)(seg_begin[cp.last] >> 56);
+      for (uint32_t i = 0; i < a.n_back; ++i) {
+        if (i == t0 && i == t1) continue;
+        const uint32_t v = a.inst[a.back[i]][cp.first];
+        if (v != a.inst[a.back[i]][cp.last]) continue;
+        const uint64_t n = a.row_ptrs[v + 1] - a.row_ptrs[v];
+        if (n <= kHashCap && (cp.want_v == ~0u || n > cp.want_n)) {
+          cp.want_v = v;
+          cp.want_n = (uint32_t)n;
+        }
+      }
+    }
+    plan[k - c_lo] = cp;
+  }
+}
+
+// Insert the n entries of list into the cuckoo tables t0 / t1 of m slots
+// each (every thread of the workgroup takes part; barriers on both sides).
+// An insertion that is still evicting after kCuckooRounds steps leaves its
+// key out and sets *fail; the tables are then not used.
+__device__ inline bool cuckoo_build(uint32_t* t0, uint32_t* t1, uint32_t m, const uint32_t* __restrict__ list, uint32_t n, uint32_t* fail) {
+  for (uint32_t i = threadIdx.x; i < m; i += kHThreads) {
+    t0[i] = kEmptySlot;
+    t1[i] = kEmptySlot;
+  }
+  if (threadIdx.x == 0) *fail = 0;
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < n; j += kHThreads) {
+    uint32_t x = list[j];
+    uint32_t t = 0;
+    bool placed = false;
+    for (int r = 0; r < kCuckooRounds && !placed; ++r) {
+      uint32_t* slot = t == 0 ? &t0[cuckoo_h0(x, m)] : &t1[cuckoo_h1(x, m)];
+      x = atomicExch(slot, x);  // the key evicted from this table moves to its slot in the other one
+      placed = x == kEmptySlot;
+      t ^= 1u;
+    }
+    if (!placed) *fail = 1;
+  }
+  __syncthreads();
+  return *fail == 0;
+}
+
+// Columns a level's verification reads per path: every backward vertex and
+// every '!=' operand, each staged once (stage = 0: more distinct columns than
+// kStageCols, never staged).
+constexpr uint32_t kStageCols = VSGM_MAX_QV - 1;
+
+struct StageCols {
+  uint32_t stage;
+  uint32_t n;
+  uint32_t col[kStageCols];       // pattern vertex of staged column k
+  uint32_t back_col[VSGM_MAX_QV]; // staged column of backward position i
+  uint32_t n_ne;
+  uint32_t ne_col[VSGM_MAX_QV];   // staged column of the k-th '!=' condition
+};
+
+// Per-wavefront window of the paths it is walking: end offset of each path
+// relative to the chunk start, packed segment begin, and the staged columns.
+constexpr uint32_t kWin = kWave;
+constexpr int kWinItems = 4;  // elements per lane per step (4: no spills at 128 VGPRs)
+struct PathWindow {
+  int32_t end[kWin];
+  int32_t start0;  // start of the window's first path, relative to the chunk
+  uint32_t n;      // paths in the window
+  uint64_t beg[kWin];
+  uint32_t col[kStageCols][kWin];
+};
+
+// Verification of up to 512 elements [e, wend) of a chunk starting at e0,
+// all in the wave's current window (paths first + i, i < win.n): element e
+// + it * 64 + lane per lane. One global load (the candidate) per element;
+// the path, its segment and bound vertices come from the window, a list
+// held in the LDS table is probed there, other lists through the membership
+// index or a binary search.
+template <int OP, int NI>
+__device__ inline uint64_t verify_window(const vsgm_level_args& a, const StageCols& sc, uint64_t first, uint64_t e0, uint64_t e,
+                                         uint64_t wend, const PathWindow& win, const HashView& hs, unsigned long long* __restrict__ path_counts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t pi[NI], c[NI], piv[NI];
+  bool ok[NI];
+  uint32_t lo = 0;
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const uint64_t x = e + (uint64_t)it * kWave + lane;
+    ok[it] = x < wend;
+    const int32_t j = (int32_t)(x - e0);
+    // first window path whose end lies beyond j; items ascend, so start at the previous item's path
+    uint32_t l = lo, h = win.n - 1;
+    if (ok[it])
+      while (l < h) {
+        const uint32_t mid = (l + h) >> 1;
+        if (win.end[mid] > j) h = mid;
+        else l = mid + 1;
+      }
+    lo = l;
+    pi[it] = l;
+    const uint64_t packed = win.beg[l];
+    piv[it] = (uint32_t)(packed >> 56);
+    const int32_t st = l ? win.end[l - 1] : win.start0;
+    c[it] = ok[it] ? a.cols[(packed & kBeginMask) + (uint64_t)(j - st)] : 0u;
+  }
+  for (uint32_t b = 0; b < a.n_back; ++b) {
+    const uint32_t* colv = win.col[sc.back_col[b]];
+    uint32_t v[NI], sl[NI], w[NI];
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      v[it] = (ok[it] && piv[it] != b) ? colv[pi[it]] : ~0u;
+      if (v[it] != ~0u && v[it] == hs.v) {
+        ok[it] = hash_contains(hs, c[it]);
+        v[it] = ~0u;
+      }
+    }
+    if (a.hub_slot) {
+#pragma unroll
+      for (int it = 0; it < NI; ++it) sl[it] = v[it] != ~0u ? a.hub_slot[v[it]] : ~0u;
+#pragma unroll
+      for (int it = 0; it < NI; ++it) w[it] = sl[it] != ~0u ? a.hub_bits[(uint64_t)sl[it] * a.hub_words + (c[it] >> 5)] : 0u;
+    } else {
+#pragma unroll
+      for (int it = 0; it < NI; ++it) sl[it] = ~0u, w[it] = 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      if (v[it] == ~0u) continue;
+      if (sl[it] != ~0u) {
+        ok[it] = (w[it] >> (c[it] & 31)) & 1u;
+      } else {
+        const uint64_t rb = a.row_ptrs[v[it]];
+        const uint32_t n = (uint32_t)(a.row_ptrs[v[it] + 1] - rb);
+        const uint32_t pos = lower_bound_u32(a.cols + rb, n, c[it]);
+        ok[it] = pos < n && a.cols[rb + pos] == c[it];
+      }
+    }
+  }
+  for (uint32_t k = 0; k < sc.n_ne; ++k) {
+    const uint32_t* colm = win.col[sc.ne_col[k]];
+#pragma unroll
+    for (int it = 0; it < NI; ++it) ok[it] = ok[it] && c[it] != colm[pi[it]];
+  }
+  uint64_t acc = 0;
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    if (!ok[it]) continue;
+    if (OP == OP_IPATH) {
+      atomicAdd(path_counts + first + pi[it], 1ull);
+    } else if (a.other >= 0) {
+      const uint64_t r = cand_row(a, (uint32_t)a.other, first + pi[it]);
+      const uint64_t cb = a.cand_off[a.other][r];
+      const uint32_t n = (uint32_t)(a.cand_off[a.other][r + 1] - cb);
+      acc += count_against(a.cand[a.other] + cb, n, c[it], a.other_op);
+    } else {
+      acc += 1;
+    }
+  }
+  return acc;
+}
+
+// One wavefront walks chunk [e0, e1) (paths first..last) alone: it loads a
+// window of up to 64 consecutive paths (one lane per path, coalesced), then
+// verifies the window's elements 512 at a time, then slides on.
+template <int OP>
+__device__ inline uint64_t walk_chunk(const vsgm_level_args& a, const StageCols& sc, const uint64_t* __restrict__ seg_begin,
+                                      const uint64_t* __restrict__ seg_off, uint64_t e0, uint64_t e1, uint64_t first, uint64_t last,
+                                      PathWindow& win, const HashView& hs, unsigned long long* __restrict__ path_counts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  uint64_t acc = 0;
+  uint64_t p = first;  // first path of the window
+  uint64_t e = e0;     // next element to verify
+  while (e < e1 && p <= last) {
+    const uint32_t n = (uint32_t)min<uint64_t>(kWin, last - p + 1);
+    if ((uint32_t)lane < n) {
+      const uint64_t q = p + lane;
+      win.end[lane] = (int32_t)((int64_t)seg_off[q + 1] - (int64_t)e0);
+      win.beg[lane] = seg_begin[q];
+      for (uint32_t k = 0; k < sc.n; ++k) win.col[k][lane] = a.inst[sc.col[k]][q];
+      if (lane == 0) {
+        win.start0 = (int32_t)((int64_t)seg_off[q] - (int64_t)e0);
+        win.n = n;
+      }
+    }
+    wave_sync();
+    const uint64_t wend = min(e1, e0 + (uint64_t)(int64_t)win.end[n - 1]);
+    for (; e < wend; e += (uint64_t)kWinItems * kWave) acc += verify_window<OP, kWinItems>(a, sc, p, e0, e, wend, win, hs, path_counts);
+    e = wend;
+    p += n;
+    wave_sync();  // the window is rewritten next
+  }
+  return acc;
+}
+
+// Register window: lane l of the wavefront holds path p + l's segment start
+// and end, packed segment begin and up to kRegCols staged columns. Each
+// 64-element block resolves its lanes' paths with a uniform loop over the
+// path boundaries that fall inside the block (readlane + select, no LDS), so
+// an element costs its candidate load, ALU work, and the probes: one LDS
+// table read when the tested list is the held one.
+constexpr uint32_t kRegCols = 3;
+constexpr int kRegItems = 4;  // elements per lane per step (4: no spills)
+
+__device__ inline uint64_t readlane_u64(uint64_t v, uint32_t l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint32_t pick3(uint32_t i, uint32_t c0, uint32_t c1, uint32_t c2) { return i == 0 ? c0 : (i == 1 ? c1 : c2); }
+
+template <int OP, int NI, int NCOL>
+__device__ inline uint64_t walk_chunk_reg(const vsgm_level_args& a, const StageCols& sc, const uint64_t* __restrict__ seg_begin,
+                                          const uint64_t* __restrict__ seg_off, uint64_t e0, uint64_t e1, uint64_t first, uint64_t last,
+                                          const HashView& hs, unsigned long long* __restrict__ path_counts) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  uint64_t acc = 0;
+  uint64_t p = first;  // first path of the window
+  uint64_t x = e0;     // next element to verify
+  while (x < e1 && p <= last) {
+    const uint32_t n = (uint32_t)min<uint64_t>(kWin, last - p + 1);
+    const bool in = lane < n;
+    const uint64_t q = p + lane;
+    const uint64_t w_start = in ? seg_off[q] : ~0ull;
+    const uint64_t w_end = in ? seg_off[q + 1] : ~0ull;
+    const uint64_t w_beg = in ? seg_begin[q] : 0;
+    const uint32_t w_c0 = (in && NCOL > 0) ? a.inst[sc.col[0]][q] : 0u;
+    const uint32_t w_c1 = (in && NCOL > 1) ? a.inst[sc.col[1]][q] : 0u;
+    const uint32_t w_c2 = (in && NCOL > 2) ? a.inst[sc.col[2]][q] : 0u;
+    const uint64_t wend = min(e1, readlane_u64(w_end, n - 1));
+    uint32_t ci = 0;  // window path containing the current block's first element
+    for (; x < wend; x += (uint64_t)NI * kWave) {
+      uint32_t c[NI], piv[NI], c0[NI], c1[NI], c2[NI], pk[NI];
+      bool ok[NI];
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        const uint64_t B = x + (uint64_t)it * kWave;
+        const uint64_t xe = B + lane;
+        ok[it] = xe < wend;
+        if (B >= wend) {
+          c[it] = piv[it] = c0[it] = c1[it] = c2[it] = pk[it] = 0;
+          continue;
+        }
+        while (readlane_u64(w_end, ci) <= B) ++ci;
+        uint64_t beg = readlane_u64(w_beg, ci), st = readlane_u64(w_start, ci);
+        uint32_t k0 = NCOL > 0 ? __builtin_amdgcn_readlane(w_c0, ci) : 0u;
+        uint32_t k1 = NCOL > 1 ? __builtin_amdgcn_readlane(w_c1, ci) : 0u;
+        uint32_t k2 = NCOL > 2 ? __builtin_amdgcn_readlane(w_c2, ci) : 0u;
+        uint32_t pi = ci;
+        for (uint32_t l = ci + 1; l < n; ++l) {
+          const uint64_t sl = readlane_u64(w_start, l);
+          if (sl >= B + kWave || sl >= wend) break;
+          if (xe >= sl) {
+            st = sl;
+            beg = readlane_u64(w_beg, l);
+            if (NCOL > 0) k0 = __builtin_amdgcn_readlane(w_c0, l);
+            if (NCOL > 1) k1 = __builtin_amdgcn_readlane(w_c1, l);
+            if (NCOL > 2) k2 = __builtin_amdgcn_readlane(w_c2, l);
+            pi = l;
+          }
+        }
+        piv[it] = (uint32_t)(beg >> 56);
+        c0[it] = k0;
+        c1[it] = NCOL > 1 ? k1 : 0u;
+        c2[it] = NCOL > 2 ? k2 : 0u;
+        pk[it] = pi;
+        c[it] = ok[it] ? a.cols[(beg & kBeginMask) + (xe - st)] : 0u;
+      }
+      for (uint32_t b = 0; b < a.n_back; ++b) {
+        const uint32_t bc = sc.back_col[b];
+        uint32_t v[NI], sl[NI], w[NI];
+        bool held[NI];
+        // index slots depend only on the path: issued before the candidates are used
+#pragma unroll
+        for (int it = 0; it < NI; ++it) {
+          v[it] = (ok[it] && piv[it] != b) ? pick3(bc, c0[it], c1[it], c2[it]) : ~0u;
+          held[it] = v[it] != ~0u && v[it] == hs.v;
+          sl[it] = (a.hub_slot && v[it] != ~0u && !held[it]) ? a.hub_slot[v[it]] : ~0u;
+        }
+#pragma unroll
+        for (int it = 0; it < NI; ++it) {
+          if (held[it]) {
+            ok[it] = hash_contains(hs, c[it]);
+            v[it] = ~0u;
+          }
+        }
+#pragma unroll
+        for (int it = 0; it < NI; ++it) w[it] = sl[it] != ~0u ? a.hub_bits[(uint64_t)sl[it] * a.hub_words + (c[it] >> 5)] : 0u;
+#pragma unroll
+        for (int it = 0; it < NI; ++it) {
+          if (v[it] == ~0u) continue;
+          if (sl[it] != ~0u) {
+            ok[it] = (w[it] >> (c[it] & 31)) & 1u;
+          } else {
+            const uint64_t rb = a.row_ptrs[v[it]];
+            const uint32_t nn = (uint32_t)(a.row_ptrs[v[it] + 1] - rb);
+            const uint32_t pos = lower_bound_u32(a.cols + rb, nn, c[it]);
+            ok[it] = pos < nn && a.cols[rb + pos] == c[it];
+          }
+        }
+      }
+      for (uint32_t k = 0; k < sc.n_ne; ++k) {
+        const uint32_t nc = sc.ne_col[k];
+#pragma unroll
+        for (int it = 0; it < NI; ++it) ok[it] = ok[it] && c[it] != pick3(nc, c0[it], c1[it], c2[it]);
+      }
+#pragma unroll
+      for (int it = 0; it < NI; ++it) {
+        if (!ok[it]) continue;
+        if (OP == OP_IPATH) {
+          atomicAdd(path_counts + p + pk[it], 1ull);
+        } else if (a.other >= 0) {
+          const uint64_t r = cand_row(a, (uint32_t)a.other, p + pk[it]);
+          const uint64_t cb = a.cand_off[a.other][r];
+          const uint32_t nn = (uint32_t)(a.cand_off[a.other][r + 1] - cb);
+          acc += count_against(a.cand[a.other] + cb, nn, c[it], a.other_op);
+        } else {
+          acc += 1;
+        }
+      }
+    }
+    x = wend;
+    p += n;
+  }
+  return acc;
+}
+
+// NCOL: the level's distinct staged columns (1..kRegCols: register window;
+// 0: more columns, LDS window).
+template <int OP, int NCOL>
+__global__ __launch_bounds__(kHThreads) void k_map_hash(vsgm_level_args a, StageCols sc, const uint64_t* __restrict__ seg_begin,
+                                                        const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t c_lo, uint64_t c_hi,
+                                                        const ChunkPlan* __restrict__ plan, uint32_t ticket_chunks,
+                                                        unsigned long long* __restrict__ tickets, unsigned long long* __restrict__ total,
+                                                        unsigned long long* __restrict__ path_counts) {
+  __shared__ uint32_t s_cuckoo[2 * kCuckooSlots];
+  __shared__ uint32_t s_fail;
+  __shared__ PathWindow s_win[NCOL ? 1 : kHWaves];
+  __shared__ ChunkPlan s_plan[kTicketCap];
+  __shared__ uint32_t s_votes[kTicketCap];
+  __shared__ uint64_t s_tmp[kHWaves];
+  __shared__ uint64_t s_ticket;
+  __shared__ uint32_t s_next;
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t n_tickets = (c_hi - c_lo + ticket_chunks - 1) / ticket_chunks;
+  uint32_t hv = ~0u, hm = 1;  // vertex whose list the cuckoo tables hold, slots in use per table
+  uint64_t acc = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_ticket = atomicAdd(tickets, 1ull);
+      s_next = 0;
+    }
+    __syncthreads();
+    const uint64_t t = s_ticket;
+    if (t >= n_tickets) break;
+    const uint64_t k_lo = c_lo + t * ticket_chunks, k_hi = min(k_lo + ticket_chunks, c_hi);
+    const uint32_t nk = (uint32_t)(k_hi - k_lo);
+    if (threadIdx.x < nk) s_plan[threadIdx.x] = plan[k_lo - c_lo + threadIdx.x];
+    __syncthreads();
+    // the list to hold for this ticket: the wanted vertex of the most chunks
+    if (threadIdx.x < nk) {
+      const uint32_t v = s_plan[threadIdx.x].want_v;
+      uint32_t votes = 0;
+      if (v != ~0u)
+        for (uint32_t i = 0; i < nk; ++i) votes += s_plan[i].want_v == v ? 1u : 0u;
+      s_votes[threadIdx.x] = votes;
+    }
+    __syncthreads();
+    uint32_t best = 0;
+    for (uint32_t i = 1; i < nk; ++i)
+      if (s_votes[i] > s_votes[best]) best = i;
+    const uint32_t want = s_votes[best] ? s_plan[best].want_v : ~0u;
+    if (want != ~0u && want != hv) {
+      const uint32_t n = s_plan[best].want_n;
+      hm = min(kCuckooSlots, max(64u, n + n / 8 + 64));  // load factor n / 2m <= 0.45 for n <= kHashCap
+      hv = cuckoo_build(s_cuckoo, s_cuckoo + kCuckooSlots, hm, a.cols + a.row_ptrs[want], n, &s_fail) ? want : ~0u;
+    }
+    const HashView hs{s_cuckoo, s_cuckoo + kCuckooSlots, hm, hv};
+    // waves take the ticket's chunks one at a time and walk them alone
+    for (;;) {
+      uint32_t j = 0;
+      if (lane == 0) j = atomicAdd(&s_next, 1u);
+      j = __shfl(j, 0);
+      if (j >= nk) break;
+      const ChunkPlan cp = s_plan[j];
+      const uint64_t k = k_lo + j;
+      const uint64_t e0 = k * kHChunk, e1 = min(e0 + kHChunk, W);
+      if constexpr (NCOL > 0)
+        acc += walk_chunk_reg<OP, kRegItems, NCOL>(a, sc, seg_begin, seg_off, e0, e1, cp.first, cp.last, hs, path_counts);
+      else
+        acc += walk_chunk<OP>(a, sc, seg_begin, seg_off, e0, e1, cp.first, cp.last, s_win[wave], hs, path_counts);
+    }
+    __syncthreads();  // every wave is done with s_plan and the table
+  }
+  if (OP != OP_IPATH) {
+    acc = wave_sum(acc);
+    if (lane == 0) s_tmp[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint64_t sum = 0;
+      for (int w = 0; w < kHWaves; ++w) sum += s_tmp[w];
+      if (sum) atomicAdd(total, (unsigned long long)sum);
+    }
+  }
+}
+
 // Scatter the valid elements of the compacting ops; chunk_off is the
 // exclusive scan of the chunk counts (n_chunks + 1 entries).
 template <int OP>
@@ -1389,7 +1894,8 @@ struct vsgm_ctx {
   std::map<void*, uint64_t> blocks;       // live block -> size class
   uint64_t cached_bytes = 0;
   uint64_t* h_pinned = nullptr;           // small pinned staging for D2H of totals
-  uint64_t* d_stats = nullptr;            // device u64[2]: scan total, adjacency degree sum
+  uint64_t* d_stats = nullptr;            // device u64[4]: scan total, adjacency degree sum, ticket counter, spare
+  int n_cu = 0;                           // compute units of the device
   std::string err;
   vsgm_op_stats stats{0, 0, 0, 0};
   // kernel profiling: HIP events around each launch on the context stream
@@ -1401,7 +1907,7 @@ struct vsgm_ctx {
   bool profiling = false;
   std::vector<KernelRecord> records;
   std::vector<hipEvent_t> spare_events;
-  vsgm_tuning tuning{0, 8192};
+  vsgm_tuning tuning{3, 8192, kDefaultColdWeight, kDefaultTicketChunks};
   // VSGM_PROGRESS=<seconds>: long map launches report their progress on
   // stderr at most this often (0: silent)
   double progress_s = 0;
@@ -1658,8 +2164,9 @@ int run_setup_scan(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t**
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stats, 2, 0);
   {
     ProfScope ps(ctx, kname<OP>(0), P * setup_bytes_per_path<OP>(a), P, P);
+    const bool hashed = (OP == OP_ICOUNT || OP == OP_IPATH) && ctx->tuning.isect_kernel == 3 && a.n_back > 1;
     hipLaunchKernelGGL(k_setup<OP>, dim3(grid_for((P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, a, *seg_begin, *seg_off,
-                       (unsigned long long*)(ctx->d_stats + 1));
+                       (unsigned long long*)(ctx->d_stats + 1), hashed ? kHashCap : 0u, ctx->tuning.cold_weight);
   }
   VSGM_TRY(launch_check(ctx, "k_setup"));
   VSGM_TRY(scan_impl(ctx, *seg_off, *seg_off, P));
@@ -1726,10 +2233,62 @@ void launch_map_window(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* 
 // The whole map of one level op, window by window. Each window is one
 // profiled launch carrying its share (by chunks) of the op's algorithmic
 // bytes `bytes`.
+// The hashed map over chunks [c_lo, c_hi) (2048-element units, as the
+// window loop counts them): the chunk plan pass, then k_map_hash on one
+// workgroup per CU pulling tickets from the context's counter.
+template <int OP>
+int launch_map_hash(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t c_lo,
+                    uint64_t c_hi, ChunkPlan* plan, uint64_t bytes, uint64_t paths, unsigned long long* total, unsigned long long* path_counts) {
+  constexpr uint64_t kRatio = kHChunk / kChunk;
+  const uint64_t h_lo = c_lo / kRatio, h_hi = (c_hi + kRatio - 1) / kRatio;
+  const uint64_t e_lo = c_lo * kChunk, e_hi = std::min(W, c_hi * kChunk);
+  hipLaunchKernelGGL(k_chunk_plan, dim3(grid_for((h_hi - h_lo + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W,
+                     h_lo, h_hi, plan);
+  VSGM_TRY(launch_check(ctx, "k_chunk_plan"));
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stats + 2, 1, 0);
+  VSGM_TRY(launch_check(ctx, "k_fill"));
+  StageCols sc{};
+  sc.stage = 1;
+  auto col_of = [&sc](uint32_t u) {
+    for (uint32_t q = 0; q < sc.n; ++q)
+      if (sc.col[q] == u) return q;
+    if (sc.n == kStageCols) {
+      sc.stage = 0;
+      return 0u;
+    }
+    sc.col[sc.n] = u;
+    return sc.n++;
+  };
+  for (uint32_t i = 0; i < a.n_back; ++i) sc.back_col[i] = col_of(a.back[i]);
+  for (uint32_t i = 0; i < a.n_cond; ++i)
+    if (a.cond[i].op == VSGM_NON_EQUAL) sc.ne_col[sc.n_ne++] = col_of(a.cond[i].operand);
+  const uint32_t tc = ctx->tuning.ticket_chunks;
+  const uint64_t n_tickets = (h_hi - h_lo + tc - 1) / tc;
+  const unsigned grid = (unsigned)std::min<uint64_t>(std::min<uint64_t>(n_tickets, (uint64_t)ctx->n_cu), ctx->tuning.max_grid);
+  {
+    ProfScope ps(ctx, kname<OP>(1), bytes, e_hi - e_lo, paths);
+    const int ncol = (sc.stage && sc.n >= 1 && sc.n <= kRegCols) ? (int)sc.n : 0;
+#define VSGM_LAUNCH_HASH(N)                                                                                                           \
+  hipLaunchKernelGGL((k_map_hash<OP, N>), dim3(grid), dim3(kHThreads), 0, ctx->stream, a, sc, seg_begin, seg_off, W, h_lo, h_hi, plan, tc, \
+                     (unsigned long long*)(ctx->d_stats + 2), total, path_counts)
+    if (ncol == 1) VSGM_LAUNCH_HASH(1);
+    else if (ncol == 2) VSGM_LAUNCH_HASH(2);
+    else if (ncol == 3) VSGM_LAUNCH_HASH(3);
+    else VSGM_LAUNCH_HASH(0);
+#undef VSGM_LAUNCH_HASH
+  }
+  return launch_check(ctx, kname<OP>(1));
+}
+
 template <int OP>
 int launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t n_chunks,
                uint64_t bytes, uint64_t* mask, uint64_t* chunk_cnt, unsigned long long* total, unsigned long long* path_counts) {
   const uint64_t n_windows = (n_chunks + kWindowChunks - 1) / kWindowChunks;
+  constexpr bool kHashable = OP == OP_ICOUNT || OP == OP_IPATH;
+  const bool hashed = kHashable && ctx->tuning.isect_kernel == 3 && a.n_back > 1;
+  Temps tmp(ctx);
+  ChunkPlan* plan = nullptr;
+  if (hashed) VSGM_TRY(tmp.get((std::min(n_chunks, kWindowChunks) * kChunk + kHChunk - 1) / kHChunk, &plan));
   const bool report = ctx->progress_s > 0 && n_windows > 1 && !ctx->capturing;
   const auto t0 = std::chrono::steady_clock::now();
   auto last = t0;
@@ -1737,11 +2296,20 @@ int launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begi
     const uint64_t c_lo = w * kWindowChunks, c_hi = std::min(n_chunks, c_lo + kWindowChunks);
     const uint64_t e_lo = c_lo * kChunk, e_hi = std::min(W, c_hi * kChunk);
     const uint64_t share = (uint64_t)((unsigned __int128)bytes * (c_hi - c_lo) / n_chunks);
-    {
-      ProfScope ps(ctx, kname<OP>(1), share, e_hi - e_lo, w == 0 ? a.path_num : 0);
-      launch_map_window<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, c_lo, c_hi, mask, chunk_cnt, total, path_counts);
+    bool done = false;
+    if constexpr (kHashable) {
+      if (hashed) {
+        VSGM_TRY(launch_map_hash<OP>(ctx, a, seg_begin, seg_off, W, c_lo, c_hi, plan, share, w == 0 ? a.path_num : 0, total, path_counts));
+        done = true;
+      }
     }
-    VSGM_TRY(launch_check(ctx, kname<OP>(1)));
+    if (!done) {
+      {
+        ProfScope ps(ctx, kname<OP>(1), share, e_hi - e_lo, w == 0 ? a.path_num : 0);
+        launch_map_window<OP>(ctx, a, seg_begin, seg_off, W, n_chunks, c_lo, c_hi, mask, chunk_cnt, total, path_counts);
+      }
+      VSGM_TRY(launch_check(ctx, kname<OP>(1)));
+    }
     if (report) {
       HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
       const auto now = std::chrono::steady_clock::now();
@@ -1885,7 +2453,8 @@ int vsgm_ctx_create(int device, uint64_t mem_budget, vsgm_ctx** out) {
   for (int i = 0; i < VSGM_TIMER_SLOTS; ++i)
     if (hipEventCreate(&ctx->ev_start[i]) != hipSuccess || hipEventCreate(&ctx->ev_stop[i]) != hipSuccess) return bail(VSGM_ERR_HIP);
   if (hipHostMalloc((void**)&ctx->h_pinned, 64, hipHostMallocDefault) != hipSuccess) return bail(VSGM_ERR_HIP);
-  if (hipMalloc((void**)&ctx->d_stats, 2 * sizeof(uint64_t)) != hipSuccess) return bail(VSGM_ERR_HIP);
+  if (hipMalloc((void**)&ctx->d_stats, 4 * sizeof(uint64_t)) != hipSuccess) return bail(VSGM_ERR_HIP);
+  if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ctx->n_cu <= 0) return bail(VSGM_ERR_HIP);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bail(VSGM_ERR_HIP);
   ctx->budget = mem_budget ? mem_budget : (uint64_t)(0.9 * (double)free_b);
@@ -2397,7 +2966,9 @@ int vsgm_split_batch(vsgm_ctx* ctx, const uint64_t* d_children, uint64_t n, uint
 
 int vsgm_set_tuning(vsgm_ctx* ctx, const vsgm_tuning* t) {
   if (!ctx || !t) return VSGM_ERR_ARG;
-  if (t->isect_kernel > 2 || t->max_grid == 0 || t->max_grid > (1u << 20)) return fail(ctx, VSGM_ERR_ARG, "bad tuning");
+  if (t->isect_kernel > 3 || t->max_grid == 0 || t->max_grid > (1u << 20)) return fail(ctx, VSGM_ERR_ARG, "bad tuning");
+  if (t->cold_weight == 0 || t->cold_weight > 1024 || t->ticket_chunks == 0 || t->ticket_chunks > (uint32_t)kTicketCap)
+    return fail(ctx, VSGM_ERR_ARG, "bad tuning: cold_weight in [1, 1024], ticket_chunks in [1, %d]", kTicketCap);
   ctx->tuning = *t;
   return VSGM_OK;
 }

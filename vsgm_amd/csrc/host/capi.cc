@@ -423,8 +423,13 @@ int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* con
                          uint64_t mem_budget, vsgm_executor** out) {
   int code = VSGM_OK;
   int s = Guard([&] {
+    VSGM_CHECK(out, "null argument");
+    VSGM_CHECK(n_plans >= 0 && (n_plans == 0 || plans), "null plans");
     std::vector<const vsgm::Plan*> ps;
-    for (int i = 0; i < n_plans; ++i) ps.push_back(plans[i]->p.get());
+    for (int i = 0; i < n_plans; ++i) {
+      VSGM_CHECK(plans[i], "null plan " + std::to_string(i));
+      ps.push_back(plans[i]->p.get());
+    }
     vsgm::ExecutorOptions opt;
     opt.mem_budget = mem_budget;
     int n_dev = 0;
@@ -448,38 +453,50 @@ int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
     else if (k == "profile") o.profile = value != 0;
     else if (k == "hub_bytes") o.hub_bytes = value < 0 ? ~0ull : (uint64_t)value;
     else if (k == "hub_min_degree") o.hub_min_degree = (uint64_t)value;
-    else if (k == "isect_kernel" || k == "max_grid") {
+    else if (k == "isect_kernel" || k == "max_grid" || k == "cold_weight" || k == "ticket_chunks") {
       // applies to every plan (the per-plan tuning of the autotuner refines it)
       vsgm_tuning t;
       vsgm_get_tuning(e->e->Context(), &t);
+      VSGM_CHECK(value >= 0 && value <= 4294967295.0, "bad tuning value for " + k);
       if (k == "isect_kernel") t.isect_kernel = (uint32_t)value;
-      else t.max_grid = (uint32_t)value;
+      else if (k == "max_grid") t.max_grid = (uint32_t)value;
+      else if (k == "cold_weight") t.cold_weight = (uint32_t)value;
+      else t.ticket_chunks = (uint32_t)value;
       if (vsgm_set_tuning(e->e->Context(), &t) != VSGM_OK) throw vsgm::Error("bad tuning value for " + k);
       for (size_t i = 0; i < e->e->GetTotalCounts().size(); ++i) {
         vsgm::PlanTuning pt = e->e->GetPlanTuning(i);
         pt.isect_kernel = t.isect_kernel;
         pt.max_grid = t.max_grid;
+        pt.cold_weight = t.cold_weight;
+        pt.ticket_chunks = t.ticket_chunks;
         e->e->SetPlanTuning(i, pt);
       }
     } else throw vsgm::Error("unknown executor option " + k);
   });
 }
 
-int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_kernel, uint32_t max_grid, uint64_t batch_bytes) {
+int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_kernel, uint32_t max_grid, uint64_t batch_bytes,
+                                  uint32_t cold_weight, uint32_t ticket_chunks) {
   return Guard([&] {
     VSGM_CHECK(e, "null argument");
-    if (isect_kernel > 2 || max_grid == 0 || max_grid > (1u << 20)) throw vsgm::Error("bad plan tuning");
-    e->e->SetPlanTuning((size_t)plan, vsgm::PlanTuning{isect_kernel, max_grid, batch_bytes});
+    VSGM_CHECK(plan >= 0 && (size_t)plan < e->e->GetTotalCounts().size(), "plan index out of range");
+    if (isect_kernel > 3 || max_grid == 0 || max_grid > (1u << 20)) throw vsgm::Error("bad plan tuning");
+    if (cold_weight == 0 || cold_weight > 1024 || ticket_chunks == 0 || ticket_chunks > 64) throw vsgm::Error("bad plan tuning");
+    e->e->SetPlanTuning((size_t)plan, vsgm::PlanTuning{isect_kernel, max_grid, batch_bytes, cold_weight, ticket_chunks});
   });
 }
 
-int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* isect_kernel, uint32_t* max_grid, uint64_t* batch_bytes) {
+int vsgm_executor_get_plan_tuning(const vsgm_executor* e, int plan, uint32_t* isect_kernel, uint32_t* max_grid, uint64_t* batch_bytes,
+                                  uint32_t* cold_weight, uint32_t* ticket_chunks) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && isect_kernel && max_grid && batch_bytes && cold_weight && ticket_chunks, "null argument");
+    VSGM_CHECK(plan >= 0 && (size_t)plan < e->e->GetTotalCounts().size(), "plan index out of range");
     const auto& t = e->e->GetPlanTuning((size_t)plan);
     *isect_kernel = t.isect_kernel;
     *max_grid = t.max_grid;
     *batch_bytes = t.batch_bytes;
+    *cold_weight = t.cold_weight;
+    *ticket_chunks = t.ticket_chunks;
   });
 }
 
@@ -509,7 +526,7 @@ int vsgm_executor_transfer_device(vsgm_executor* e, const uint32_t* sources, uin
 int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree,
                                   int* index) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && (sources || n_sources == 0), "null argument");
     std::vector<vsgm::uintV> s(sources, sources + n_sources);
     size_t i = e->e->AddDeviceView(s, hop, root_degree);
     if (index) *index = (int)i;
@@ -533,14 +550,14 @@ int vsgm_executor_view_csr(const vsgm_executor* e, int view, uint64_t* row_ptrs,
 
 int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && n_plans >= 0, "bad arguments");
     e->e->MatchAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{});
   });
 }
 
 int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint64_t* n_nodes) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && n_plans >= 0, "bad arguments");
     uint64_t n = e->e->CaptureAll(plans ? std::vector<int>(plans, plans + n_plans) : std::vector<int>{});
     if (n_nodes) *n_nodes = n;
   });
@@ -557,7 +574,7 @@ int vsgm_executor_replay(vsgm_executor* e, double* ms) {
 
 int vsgm_executor_hub_count(const vsgm_executor* e, uint32_t* n) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && n, "null argument");
     *n = e->e->HubCount();
   });
 }
@@ -571,7 +588,7 @@ int vsgm_executor_transfer(vsgm_executor* e, const vsgm_view_holder* h) {
 
 int vsgm_executor_transfer_graph(vsgm_executor* e, const vsgm_graph* g, const uint32_t* sources, uint64_t n_sources) {
   return Guard([&] {
-    VSGM_CHECK(e && g, "null argument");
+    VSGM_CHECK(e && g && (sources || n_sources == 0), "null argument");
     std::vector<vsgm::uintV> s;
     if (sources) s.assign(sources, sources + n_sources);
     e->e->TransferGraph(g->g, s);
@@ -587,7 +604,7 @@ int vsgm_executor_match(vsgm_executor* e, int do_match) {
 
 int vsgm_executor_counts(const vsgm_executor* e, uint64_t* counts, int n) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && counts, "null argument");
     const auto& c = e->e->GetTotalCounts();
     VSGM_CHECK(n >= (int)c.size(), "counts buffer too small");
     for (size_t i = 0; i < c.size(); ++i) counts[i] = c[i];
@@ -596,7 +613,7 @@ int vsgm_executor_counts(const vsgm_executor* e, uint64_t* counts, int n) {
 
 int vsgm_executor_match_ms(const vsgm_executor* e, double* ms, int n) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && ms, "null argument");
     const auto& c = e->e->GetMatchMs();
     VSGM_CHECK(n >= (int)c.size(), "buffer too small");
     for (size_t i = 0; i < c.size(); ++i) ms[i] = c[i];
@@ -612,7 +629,7 @@ int vsgm_executor_reset(vsgm_executor* e) {
 
 int vsgm_executor_level_stats(const vsgm_executor* e, int plan, vsgm_level_stat* out, int cap, int* n_out) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && n_out && (cap <= 0 || out), "null argument");
     const auto& all = e->e->GetLevelStats();
     VSGM_CHECK(plan >= 0 && plan < (int)all.size(), "plan index out of range");
     const auto& s = all[plan];
@@ -630,7 +647,7 @@ int vsgm_executor_context(vsgm_executor* e, vsgm_ctx** ctx) {
 
 int vsgm_executor_peak_bytes(const vsgm_executor* e, uint64_t* bytes) {
   return Guard([&] {
-    VSGM_CHECK(e, "null argument");
+    VSGM_CHECK(e && bytes, "null argument");
     *bytes = e->e->PeakDeviceBytes();
   });
 }

@@ -124,14 +124,15 @@ void PipelineExecutor::SetPlans(std::vector<const Plan*> plans) {
 
 void PipelineExecutor::SetPlanTuning(size_t plan, const PlanTuning& t) {
   VSGM_CHECK(plan < plans_.size(), "plan index out of range");
-  VSGM_CHECK(t.isect_kernel <= 2 && t.max_grid > 0, "bad plan tuning");
+  VSGM_CHECK(t.isect_kernel <= 3 && t.max_grid > 0 && t.max_grid <= (1u << 20), "bad plan tuning");
+  VSGM_CHECK(t.cold_weight >= 1 && t.cold_weight <= 1024 && t.ticket_chunks >= 1 && t.ticket_chunks <= 64, "bad plan tuning");
   ReleaseGraph();
   tuning_[plan] = t;
 }
 
 void PipelineExecutor::ApplyTuning(size_t plan) {
   const PlanTuning& t = tuning_[plan];
-  vsgm_tuning kt{t.isect_kernel, t.max_grid};
+  vsgm_tuning kt{t.isect_kernel, t.max_grid, t.cold_weight, t.ticket_chunks};
   Check(vsgm_set_tuning(ctx_, &kt), "tuning");
   batch_override_ = t.batch_bytes;
 }
@@ -164,8 +165,18 @@ void PipelineExecutor::ClearViews() {
 // Upload one view (row_ptrs over the full vertex range, cols, sources) and
 // index its highest-degree lists: vertices of degree >= hub_min_degree,
 // largest first, as many as hub_limit bytes hold.
+// Every source becomes a level-0 binding whose row_ptrs the kernels read, so
+// ids outside the vertex range are refused before any upload or launch.
+void PipelineExecutor::CheckSources(const uint32_t* sources, uint64_t n_sources) const {
+  VSGM_CHECK(n_sources == 0 || sources, "null sources");
+  for (uint64_t i = 0; i < n_sources; ++i)
+    VSGM_CHECK(sources[i] < vertex_count_, "source " + std::to_string(sources[i]) + " (position " + std::to_string(i) +
+                                               ") is not a vertex of the " + std::to_string(vertex_count_) + "-vertex graph");
+}
+
 PipelineExecutor::View PipelineExecutor::UploadView(const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n_cols, const uint32_t* sources,
                                                     uint64_t n_sources, uint64_t hub_limit) {
+  CheckSources(sources, n_sources);
   View v;
   v.row_ptrs = std::make_shared<DeviceBuffer>(ctx_, (vertex_count_ + 1) * sizeof(uintE));
   v.cols = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(n_cols, 1) * sizeof(uintV));
@@ -226,6 +237,7 @@ void PipelineExecutor::TransferDevice(const std::vector<uintV>& sources, int hop
 PipelineExecutor::View PipelineExecutor::BuildDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree,
                                                          uint64_t hub_limit) {
   VSGM_CHECK(graph_rp_ && graph_cols_, "device view before SetGraph");
+  CheckSources(sources.data(), sources.size());
   View v;
   v.sources = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources.size(), 1) * sizeof(uintV));
   v.sources_num = sources.size();

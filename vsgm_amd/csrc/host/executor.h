@@ -47,9 +47,11 @@ class DeviceBuffer;
 
 // Launch-shape and budget choices for one plan (set by the autotuner).
 struct PlanTuning {
-  uint32_t isect_kernel = 0;
+  uint32_t isect_kernel = 3;
   uint32_t max_grid = 8192;
-  uint64_t batch_bytes = 0;  // 0: the executor's option
+  uint64_t batch_bytes = 0;     // 0: the executor's option
+  uint32_t cold_weight = 2;     // isect_kernel 3 pivot cost of a cold list test
+  uint32_t ticket_chunks = 64;  // isect_kernel 3 chunks per ticket
 };
 
 class PipelineExecutor {
@@ -123,6 +125,7 @@ class PipelineExecutor {
   void RunBatch(size_t level, const Level& slice);
   uint64_t LevelBatchBytes(size_t level);
   void FillArgs(size_t level, const Level& l, vsgm_level_args& a) const;
+  void CheckSources(const uint32_t* sources, uint64_t n_sources) const;
   void Check(int status, const char* what) const;
   void ApplyTuning(size_t plan);
 

@@ -153,10 +153,45 @@ int Run(int argc, char** argv) {
     for (uintV v : pool[bin]->GetSources()) h = (h ^ v) * 1099511628211ull;
     return h;
   };
+  // The checkpoint belongs to one run: its first line carries a key over
+  // every serialized plan, the graph (vertex count, entry count and degree
+  // sequence), the hop and the root-degree filter. A file written under
+  // another key (a different query, graph or view) is refused rather than
+  // mixed into these totals.
+  auto run_key = [&] {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](uint64_t x) {
+      for (int b = 0; b < 8; ++b) h = (h ^ ((x >> (8 * b)) & 0xFF)) * 1099511628211ull;
+    };
+    mix(plans.size());
+    for (auto& p : plans) {
+      const std::string t = p->Serialize();
+      mix(t.size());
+      for (unsigned char ch : t) h = (h ^ ch) * 1099511628211ull;
+    }
+    mix(graph.vertex_count);
+    mix(graph.cols.size());
+    for (uint64_t x : graph.row_ptrs) mix(x);
+    mix((uint64_t)hop);
+    mix(root_degree);
+    return h;
+  };
   std::map<int, std::vector<uintC>> done;
+  bool ck_has_header = false;
+  const uint64_t ck_key = ck_file.empty() ? 0 : run_key();
   if (!ck_file.empty()) {
     std::ifstream in(ck_file);
     std::string line;
+    if (in && std::getline(in, line)) {
+      std::istringstream hs(line);
+      std::string tag;
+      uint64_t key = 0;
+      if (!(hs >> tag >> std::hex >> key) || tag != "vsgm-checkpoint")
+        throw Error("checkpoint " + ck_file + " has no vsgm-checkpoint header line; refusing to mix it into this run");
+      if (key != ck_key)
+        throw Error("checkpoint " + ck_file + " was written for another run (query, graph, hop or root-degree filter differ)");
+      ck_has_header = true;
+    }
     while (std::getline(in, line)) {
       std::istringstream ls(line);
       long bin = -1;
@@ -174,6 +209,10 @@ int Run(int argc, char** argv) {
   if (!ck_file.empty()) {
     ck.open(ck_file, std::ios::app);
     if (!ck) throw Error("cannot append to " + ck_file);
+    if (!ck_has_header) {
+      ck << "vsgm-checkpoint " << std::hex << ck_key << std::dec << "\n" << std::flush;
+      if (!ck) throw Error("write to " + ck_file + " failed");
+    }
   }
   std::mutex ck_mu;
   auto record = [&](int bin, const std::vector<uintC>& before, const std::vector<uintC>& after) {

@@ -76,6 +76,10 @@ class HubIndex(C.Structure):
     _fields_ = [("slot", C.c_void_p), ("bits", C.c_void_p), ("words", C.c_uint64), ("n_hubs", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("isect_kernel", C.c_uint32), ("max_grid", C.c_uint32), ("cold_weight", C.c_uint32), ("ticket_chunks", C.c_uint32)]
+
+
 def _declare(L):
     P, u64, i = C.c_void_p, C.c_uint64, C.c_int
     sig = {
@@ -106,6 +110,8 @@ def _declare(L):
         "vsgm_hub_index_build": ([P, P, P, u64, P, C.c_uint32, C.POINTER(HubIndex)], i),
         "vsgm_hub_index_free": ([P, C.POINTER(HubIndex)], i),
         "vsgm_csr_from_pairs": ([P, P, u64, u64, i, i, C.POINTER(ViewOut)], i),
+        "vsgm_set_tuning": ([P, C.POINTER(Tuning)], i),
+        "vsgm_get_tuning": ([P, C.POINTER(Tuning)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -175,6 +181,17 @@ class DeviceContext:
 
     def sync(self):
         self.check(hiplib().vsgm_ctx_sync(self.h), "sync")
+
+    def tuning(self, **knobs) -> dict:
+        """Set any of isect_kernel, max_grid, cold_weight, ticket_chunks
+        (vsgm_tuning); returns the setting in force."""
+        t = Tuning()
+        self.check(hiplib().vsgm_get_tuning(self.h, C.byref(t)), "get_tuning")
+        for k, v in knobs.items():
+            setattr(t, k, v)
+        if knobs:
+            self.check(hiplib().vsgm_set_tuning(self.h, C.byref(t)), "set_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def mem_info(self):
         a, b, p = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -1,8 +1,9 @@
 """All connected non-isomorphic simple graphs on 3..6 vertices (2+6+21+112 = 141).
 
 Canonical form = the smallest edge bitmask over all vertex permutations,
-computed by brute force with numpy. Used by the plan-fixture generator, the
-planner tests and the random pattern mix of the batched workload.
+computed by brute force with numpy. The random pattern mix of the batched
+workload draws from these, and the learned planner's training records cover
+them; the plan-fixture generator and the tests enumerate them too.
 """
 from __future__ import annotations
 
