@@ -111,11 +111,11 @@ def test_autotuner_installs_a_config_and_keeps_counts(vsgm, setup):
     g, plans, want = setup
     ex = vsgm.PipelineExecutor(0, g.vertex_count, plans)
     ex.transfer_graph(g)
-    cands = {"isect_kernel": [0, 1, 2], "max_grid": [1024, 8192], "batch_bytes": [0, 1 << 16]}
+    cands = {"isect_kernel": [3, 0, 2], "max_grid": [1024, 8192], "batch_bytes": [0, 1 << 16], "ticket_chunks": [16, 64]}
     print("[gpu] autotune 3 plans", flush=True)
     res = autotune.tune(ex, plans=[0, len(plans) - 2, len(plans) - 1], repeats=1, candidates=cands)
     for p, r in res.items():
-        assert r["trials"] == 5  # the default plus every other value of each knob
+        assert r["trials"] == 6  # the default plus every other value of each knob
         assert r["ms"] <= r["default_ms"]
         assert ex.plan_tuning(p) == r["config"]
         assert all(r["config"][k] in v for k, v in cands.items())
