@@ -295,6 +295,20 @@ int vsgm_intersect_write(vsgm_ctx* ctx, const vsgm_level_args* a, vsgm_csr_out* 
  * other >= 0, of the number of other-candidates c2 with "cand other_op c2"
  * (ThreadCountWithComputedVertex, gpu_check_constraints_common.h:120-139). */
 int vsgm_intersect_count_total(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_total);
+/* Allocation-free COMPUTE_COUNT. No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture
+ * by the caller, tight serving loops, harnesses that forbid hipMalloc).
+ * Same result as vsgm_intersect_count_total (same kernels and tuning), but
+ * every temporary (segment words, scans, chunk plans) comes from a 64 MiB
+ * scratch array the library holds on each device for the life of the
+ * process: no hipMalloc / hipFree, no pool block, no event. Limit:
+ * path_num <= VSGM_NOALLOC_MAX_PATHS (VSGM_ERR_ARG beyond). Calls on one
+ * device share that scratch, so calls that overlap on one device (several
+ * contexts or threads at once) race; calls in sequence are safe (each
+ * rewrites every scratch word it reads). Not available while the context
+ * records a capture (vsgm_capture_begin). */
+#define VSGM_NOALLOC_MAX_PATHS (1ull << 21)
+int vsgm_intersect_count_total_noalloc(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_total);
 /* Per-path intersection counts (the count pass of GpsmBinSearch and
  * ComputePathCount, src/executor/ops/compute_path_count.h:7-49):
  * d_counts[p] = number of valid candidates of path p. */

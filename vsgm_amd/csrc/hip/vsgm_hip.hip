@@ -1882,6 +1882,15 @@ __global__ __launch_bounds__(kBlock) void k_split(const uint64_t* __restrict__ p
 // ---------------------------------------------------------------------------
 // context and pool
 
+// Device scratch of the allocation-free entry point
+// (vsgm_intersect_count_total_noalloc): every device this library runs on
+// holds one copy for the life of the process. Host code never passes this
+// name to a kernel (on the host it is only a stand-in for the symbol);
+// vsgm_ctx_create looks up the device address once per context, with the
+// context's device current, and everything uses that address.
+constexpr uint64_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 struct vsgm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1895,6 +1904,11 @@ struct vsgm_ctx {
   uint64_t cached_bytes = 0;
   uint64_t* h_pinned = nullptr;           // small pinned staging for D2H of totals
   uint64_t* d_stats = nullptr;            // device u64[4]: scan total, adjacency degree sum, ticket counter, spare
+  // device address of `scratch` on this context's device, and the bump arena
+  // that pool temporaries come from while an allocation-free call runs
+  uint8_t* scratch_ptr = nullptr;
+  bool arena_on = false;
+  uint64_t arena_off = 0;
   int n_cu = 0;                           // compute units of the device
   std::string err;
   vsgm_op_stats stats{0, 0, 0, 0};
@@ -2010,6 +2024,9 @@ int pool_free(vsgm_ctx* ctx, void* p) {
 }
 
 // RAII holder of pool temporaries inside one entry point.
+// While ctx->arena_on, temporaries are carved from the context's scratch
+// (256-byte aligned, bump order, never freed individually) and the pool is
+// not touched.
 struct Temps {
   vsgm_ctx* ctx;
   std::vector<void*> ptrs;
@@ -2019,8 +2036,18 @@ struct Temps {
   }
   template <typename T>
   int get(uint64_t n, T** out) {
+    const uint64_t bytes = std::max<uint64_t>(n, 1) * sizeof(T);
+    if (ctx->arena_on) {
+      const uint64_t off = (ctx->arena_off + 255) & ~255ull;
+      if (off + bytes > kScratchBytes)
+        return fail(ctx, VSGM_ERR_BUDGET, "temporaries of %llu bytes exceed the %llu-byte scratch", (unsigned long long)(off + bytes),
+                    (unsigned long long)kScratchBytes);
+      ctx->arena_off = off + bytes;
+      *out = (T*)(ctx->scratch_ptr + off);
+      return VSGM_OK;
+    }
     void* p = nullptr;
-    int s = pool_alloc(ctx, std::max<uint64_t>(n, 1) * sizeof(T), &p);
+    int s = pool_alloc(ctx, bytes, &p);
     if (s != VSGM_OK) return s;
     ptrs.push_back(p);
     *out = (T*)p;
@@ -2455,6 +2482,10 @@ int vsgm_ctx_create(int device, uint64_t mem_budget, vsgm_ctx** out) {
   if (hipHostMalloc((void**)&ctx->h_pinned, 64, hipHostMallocDefault) != hipSuccess) return bail(VSGM_ERR_HIP);
   if (hipMalloc((void**)&ctx->d_stats, 4 * sizeof(uint64_t)) != hipSuccess) return bail(VSGM_ERR_HIP);
   if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ctx->n_cu <= 0) return bail(VSGM_ERR_HIP);
+  if (hipGetSymbolAddress((void**)&ctx->scratch_ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ctx->scratch_ptr) {
+    (void)hipGetLastError();
+    return bail(VSGM_ERR_HIP);
+  }
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bail(VSGM_ERR_HIP);
   ctx->budget = mem_budget ? mem_budget : (uint64_t)(0.9 * (double)free_b);
@@ -2824,6 +2855,26 @@ int vsgm_intersect_count_total(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t
   VSGM_TRY(check_intersect(ctx, a));
   if (a->other >= 0) VSGM_TRY(check_cand(ctx, a, (uint32_t)a->other));
   return run_count<OP_ICOUNT>(ctx, *a, d_total, nullptr);
+}
+
+int vsgm_intersect_count_total_noalloc(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_total) {
+  VSGM_TRY(check_args(ctx, a));
+  if (!d_total) return fail(ctx, VSGM_ERR_ARG, "null total");
+  VSGM_TRY(check_intersect(ctx, a));
+  if (a->other >= 0) VSGM_TRY(check_cand(ctx, a, (uint32_t)a->other));
+  if (a->path_num > VSGM_NOALLOC_MAX_PATHS)
+    return fail(ctx, VSGM_ERR_ARG, "path_num %llu exceeds the allocation-free limit of %llu paths", (unsigned long long)a->path_num,
+                (unsigned long long)VSGM_NOALLOC_MAX_PATHS);
+  if (ctx->capturing) return fail(ctx, VSGM_ERR_ARG, "the allocation-free count is not available during graph capture");
+  // no profiling events either: the call creates nothing
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  ctx->arena_on = true;
+  ctx->arena_off = 0;
+  const int s = run_count<OP_ICOUNT>(ctx, *a, d_total, nullptr);
+  ctx->arena_on = false;
+  ctx->profiling = prof;
+  return s;
 }
 
 int vsgm_intersect_count_paths(vsgm_ctx* ctx, const vsgm_level_args* a, uint64_t* d_counts) {

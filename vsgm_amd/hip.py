@@ -100,6 +100,7 @@ def _declare(L):
         "vsgm_fill_u64": ([P, P, u64, u64], i),
         "vsgm_intersect_write": ([P, C.POINTER(LevelArgs), C.POINTER(CsrOut)], i),
         "vsgm_intersect_count_total": ([P, C.POINTER(LevelArgs), P], i),
+        "vsgm_intersect_count_total_noalloc": ([P, C.POINTER(LevelArgs), P], i),
         "vsgm_intersect_count_paths": ([P, C.POINTER(LevelArgs), P], i),
         "vsgm_expand": ([P, C.POINTER(LevelArgs), P, C.c_uint32, P, C.c_uint32, C.POINTER(ExpandOut)], i),
         "vsgm_filter": ([P, C.POINTER(LevelArgs), C.POINTER(CsrOut)], i),
