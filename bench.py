@@ -121,6 +121,37 @@ def cpu_baseline(O, rp, cl, plan_texts, budget_s, threads, seed=12345):
         m *= 4
 
 
+def schedule_steps(run_first, run_step, restart, warmup, steps, budget_s, now, reserve=20.0):
+    """Warm-up and timed steps within budget_s seconds of the clock `now`.
+    The first full step is timed like every other; it counts as a warm-up
+    step only when one more full step still fits after it, else it is the
+    one timed step (so a timed step always exists). Timed steps come first
+    in the budget: further warm-up steps (at most `warmup` in all) run only
+    in the time left once all `steps` timed steps are provided for; timed
+    steps then run while one more still fits (at most `steps`).
+    `restart` resets the counts and the kernel records before the timed
+    steps. Returns (warm-up steps, timed steps, t0, t1), t0 / t1 bracketing
+    the timed steps on the same clock."""
+    restart()
+    t0 = now()
+    est = run_first("step 0")
+    timed, warm = 1, 0
+    if warmup > 0 and now() + est + reserve <= budget_s:
+        warm = 1
+        spare = int((budget_s - reserve - now()) // est) - max(steps, 1) if est > 0 else warmup
+        while warm < warmup and spare > 0:
+            est = run_step(f"warm-up step {warm}")
+            warm += 1
+            spare -= 1
+        restart()
+        t0 = now()
+        est = run_step("timed step 0")
+    while timed < max(steps, 1) and now() + est + reserve <= budget_s:
+        est = run_step(f"timed step {timed}")
+        timed += 1
+    return warm, timed, t0, now()
+
+
 def load_traffic(kernel, workload, plan, isect_kernel):
     """Measured traffic of `kernel` on this exact configuration, or None: the
     profile file must name the same workload, plan, map kernel variant and
@@ -152,7 +183,7 @@ def main():
     ap.add_argument("--hub-bytes", type=float, default=-1, help="membership index bytes (-1 auto, 0 off)")
     ap.add_argument("--hub-min-degree", type=int, default=64)
     ap.add_argument("--isect-kernel", type=int, default=3)
-    ap.add_argument("--cold-weight", type=int, default=2)
+    ap.add_argument("--cold-weight", type=int, default=1)
     ap.add_argument("--ticket-chunks", type=int, default=64)
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
     ap.add_argument("--cl-vertices", type=int, default=65_600_000)
@@ -248,37 +279,29 @@ def main():
         log(f"sampled-root parity: gpu {sample_check['gpu']} oracle {cpu_counts}")
         assert sample_check["equal"], f"GPU counts {sample_check['gpu']} differ from the oracle {cpu_counts} on the sampled roots"
 
-    # warm-up: only while a timed step would still fit after it
-    reserve = 15.0  # line assembly and process exit
-    est = None
-    warm = 0
-    for w in range(args.warmup):
-        if est is not None and elapsed() + 2 * est + reserve > args.budget_s:
-            break
-        t = time.perf_counter()
-        ex.match()
-        est = time.perf_counter() - t
-        warm += 1
-        log(f"warmup {w}: {est:.2f}s counts={ex.total_counts}")
-    if est is not None and elapsed() + est + reserve > args.budget_s:
-        log(f"warm-up step took {est:.1f}s; one timed step still runs (every pattern gets one)")
-    ref_counts = [c // warm for c in ex.total_counts] if warm else None
-
-    ex.reset()
-    ex.profile(True)
-    t0 = time.perf_counter()
-    steps = 0
-    while steps < max(args.steps, 1):
-        if steps >= 1 and elapsed() + est + reserve > args.budget_s:
-            break
+    def run_step(label):
         ts = time.perf_counter()
         ex.match()
-        est = time.perf_counter() - ts
-        steps += 1
-        log(f"step {steps - 1}: {est:.2f}s")
-    t1 = time.perf_counter()
+        dt = time.perf_counter() - ts
+        log(f"{label}: {dt:.2f}s counts={ex.total_counts}")
+        return dt
+
+    def restart():
+        ex.reset()
+        ex.profile(True)  # drops the records of earlier steps
+
+    first_counts = []
+
+    def run_first(label):
+        dt = run_step(label)
+        first_counts.extend(ex.total_counts)
+        return dt
+
+    warm, steps, t0, t1 = schedule_steps(run_first, run_step, restart, args.warmup, args.steps, args.budget_s, elapsed)
     kstats = ex.kernel_stats()
     ex.profile(False)
+    log(f"{warm} warm-up and {steps} timed steps (requested {args.warmup} and {args.steps}) within the {args.budget_s:.0f}s budget")
+    ref_counts = first_counts if warm else None
 
     counts = [c // steps for c in ex.total_counts]
     assert all(c * steps == tc for c, tc in zip(counts, ex.total_counts)), "counts differ between steps"
@@ -628,7 +651,7 @@ def batched_main(args):
         bx.replay()
         run_s = time.perf_counter() - t
         counts = bx.total_counts
-        log(f"{label}: tuned in {tune_s:.1f}s ({sum(r['config'] != {'isect_kernel': 3, 'max_grid': 8192, 'batch_bytes': 0, 'cold_weight': 2, 'ticket_chunks': 64} for r in res.values())} "
+        log(f"{label}: tuned in {tune_s:.1f}s ({sum(r['config'] != {'isect_kernel': 3, 'max_grid': 8192, 'batch_bytes': 0, 'cold_weight': 1, 'ticket_chunks': 64} for r in res.values())} "
             f"plans off the default), captured {nodes} nodes in {cap_s:.1f}s, replay {run_s:.3f}s")
         return counts, run_s, nodes
 

@@ -25,3 +25,43 @@ def test_intersection_rates_count_probes_per_backward_list(vsgm):
     sets = 7 * 1 * 0 + 7 * 3 * 1 + 7 * 5 * 2
     assert r == {"probes_per_s": probes / 0.5, "set_intersections_per_s": sets / 0.5}
     assert bench.intersection_rates(p.serialize(), p.ops, levels, 0.0) is None
+
+
+class _Clock:
+    def __init__(self, step_s, start=0.0):
+        self.t, self.step_s, self.runs, self.restarts = start, step_s, [], 0
+
+    def now(self):
+        return self.t
+
+    def step(self, label):
+        self.t += self.step_s
+        self.runs.append(label)
+        return self.step_s
+
+    def restart(self):
+        self.restarts += 1
+
+
+def test_budget_with_a_step_longer_than_half_the_budget():
+    # one 300 s step after 40 s of set-up: no room for a warm-up, one timed step
+    c = _Clock(300.0, start=40.0)
+    warm, timed, t0, t1 = bench.schedule_steps(c.step, c.step, c.restart, 5, 20, 480.0, c.now)
+    assert (warm, timed) == (0, 1) and t1 - t0 == 300.0 and c.runs == ["step 0"] and c.t <= 480.0
+
+
+def test_budget_warm_up_then_timed_steps():
+    # 100 s steps after 40 s: step 0 becomes the warm-up, then timed steps while one more fits
+    c = _Clock(100.0, start=40.0)
+    warm, timed, t0, t1 = bench.schedule_steps(c.step, c.step, c.restart, 5, 20, 480.0, c.now)
+    assert warm == 1 and timed == 3 and c.t <= 480.0 and t1 - t0 == 300.0 and c.restarts == 2
+    # short steps: the requested counts are upper bounds that are reached
+    c = _Clock(1.0)
+    warm, timed, _, _ = bench.schedule_steps(c.step, c.step, c.restart, 5, 20, 480.0, c.now)
+    assert (warm, timed) == (5, 20) and len(c.runs) == 25
+
+
+def test_budget_without_warm_up():
+    c = _Clock(50.0)
+    warm, timed, t0, t1 = bench.schedule_steps(c.step, c.step, c.restart, 0, 3, 480.0, c.now)
+    assert (warm, timed) == (0, 3) and t1 - t0 == 150.0

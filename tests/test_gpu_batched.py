@@ -67,7 +67,7 @@ def test_plan_tuning_keeps_counts(vsgm, setup):
     ex = vsgm.PipelineExecutor(0, g.vertex_count, plans)
     for i in range(len(plans)):
         ex.set_plan_tuning(i, isect_kernel=i % 3, max_grid=(256, 2048, 8192)[i % 3], batch_bytes=(0, 1 << 16)[i % 2])
-    assert ex.plan_tuning(4) == {"isect_kernel": 1, "max_grid": 2048, "batch_bytes": 0, "cold_weight": 2, "ticket_chunks": 64}
+    assert ex.plan_tuning(4) == {"isect_kernel": 1, "max_grid": 2048, "batch_bytes": 0, "cold_weight": 1, "ticket_chunks": 64}
     ex.transfer_graph(g)
     print("[gpu] per-plan tuning", flush=True)
     ex.match()

@@ -136,7 +136,7 @@ def test_tuning_bounds(vsgm, gpu):
     from vsgm_amd.hip import DeviceContext
 
     ctx = DeviceContext(0)
-    assert ctx.tuning() == {"isect_kernel": 3, "max_grid": 8192, "cold_weight": 2, "ticket_chunks": 64}
+    assert ctx.tuning() == {"isect_kernel": 3, "max_grid": 8192, "cold_weight": 1, "ticket_chunks": 64}
     for bad in ({"isect_kernel": 4}, {"ticket_chunks": 0}, {"ticket_chunks": 65}, {"cold_weight": 0}):
         with pytest.raises(Exception):
             ctx.tuning(**bad)

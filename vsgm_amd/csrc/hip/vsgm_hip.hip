@@ -53,7 +53,7 @@ constexpr int kMaxGrid = 8192;
 // entries keep the load factor <= 0.45.
 constexpr uint32_t kCuckooSlots = 14336;
 constexpr uint32_t kHashCap = 12800;
-constexpr uint32_t kDefaultColdWeight = 2;
+constexpr uint32_t kDefaultColdWeight = 1;
 constexpr uint32_t kDefaultTicketChunks = 64;
 
 enum OpKind { OP_ICOUNT = 0, OP_IWRITE = 1, OP_EXPAND = 2, OP_FILTER = 3, OP_COUNT2 = 4, OP_IPATH = 5 };
@@ -1550,13 +1550,19 @@ __device__ inline uint64_t walk_chunk(const vsgm_level_args& a, const StageCols&
 }
 
 // Register window: lane l of the wavefront holds path p + l's segment start
-// and end, packed segment begin and up to kRegCols staged columns. Each
-// 64-element block resolves its lanes' paths with a uniform loop over the
-// path boundaries that fall inside the block (readlane + select, no LDS), so
-// an element costs its candidate load, ALU work, and the probes: one LDS
-// table read when the tested list is the held one.
+// and end, packed segment begin and up to kRegCols staged columns. A path
+// with at least kUniTail elements left is verified in wave-uniform steps
+// (its data in scalar registers, up to kUniLarge candidate loads in flight
+// per lane, a masked step for its tail). Shorter stretches go 64-element
+// block by block: each block resolves its lanes' paths with a uniform loop
+// over the path boundaries inside it (readlane + select, no LDS). Either
+// way an element costs its candidate load, ALU work and its probes: one
+// pair of LDS reads when the tested list is the held one.
 constexpr uint32_t kRegCols = 3;
 constexpr int kRegItems = 4;  // elements per lane per step (4: no spills)
+constexpr int kUniLarge = 16;  // loads per lane of a wave-uniform step over a long path
+constexpr int kUniSmall = 8;
+constexpr uint64_t kUniTail = 64;  // path elements from which a masked uniform step beats the per-lane path walk
 
 __device__ inline uint64_t readlane_u64(uint64_t v, uint32_t l) {
   const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
@@ -1565,6 +1571,114 @@ __device__ inline uint64_t readlane_u64(uint64_t v, uint32_t l) {
 }
 
 __device__ inline uint32_t pick3(uint32_t i, uint32_t c0, uint32_t c1, uint32_t c2) { return i == 0 ? c0 : (i == 1 ? c1 : c2); }
+
+// Wave-uniform steps over one path: while at least UI * 64 of its elements
+// remain, the path's segment, bound vertices and probe targets are uniform
+// (scalar registers) and each lane keeps UI candidate loads in flight.
+// Verifies elements [x, x + k * UI * 64) for the largest such k; with TAIL,
+// one more step over the rest [x, pend) (at most UI * 64 elements, lanes
+// past pend idle). Returns the count contribution and advances x.
+template <int OP, int UI, int NCOL, bool TAIL = false>
+__device__ inline uint64_t uniform_steps(const vsgm_level_args& a, const StageCols& sc, uint64_t q, uint64_t st, uint64_t packed, uint32_t k0,
+                                         uint32_t k1, uint32_t k2, uint64_t pend, uint64_t& x, const HashView& hs,
+                                         unsigned long long* __restrict__ path_counts) {
+  constexpr uint64_t kStep = (uint64_t)UI * kWave;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  if (TAIL ? (pend <= x) : (pend - x < kStep)) return 0;
+  const uint32_t piv = (uint32_t)(packed >> 56);
+  const uint32_t* seg = a.cols + (packed & kBeginMask) - st;  // element y's candidate is seg[y]
+  // probe plan of the tested lists (wave-uniform): 0 held in LDS, 1 index row, 2 binary search
+  // (a level of this kernel has at most kRegCols backward lists: they are
+  // distinct staged columns; loops run to the constant bound so the plan
+  // stays in registers)
+  uint32_t mode[kRegCols];
+  const uint32_t* tgt[kRegCols];
+  uint32_t tlen[kRegCols];
+#pragma unroll
+  for (uint32_t b = 0; b < kRegCols; ++b) {
+    mode[b] = 3;  // the pivot, or no list: not tested
+    tgt[b] = nullptr;
+    tlen[b] = 0;
+    if (b >= a.n_back || b == piv) continue;
+    const uint32_t v = pick3(sc.back_col[b], k0, k1, k2);
+    if (v == hs.v) {
+      mode[b] = 0;
+      continue;
+    }
+    const uint32_t sl = a.hub_slot ? a.hub_slot[v] : ~0u;
+    if (sl != ~0u) {
+      mode[b] = 1;
+      tgt[b] = a.hub_bits + (uint64_t)sl * a.hub_words;
+    } else {
+      mode[b] = 2;
+      const uint64_t rb = a.row_ptrs[v];
+      tgt[b] = a.cols + rb;
+      tlen[b] = (uint32_t)(a.row_ptrs[v + 1] - rb);
+    }
+  }
+  uint64_t acc = 0;
+  for (; TAIL ? (x < pend) : (pend - x >= kStep); x += kStep) {
+    uint32_t c[UI];
+    bool ok[UI];
+#pragma unroll
+    for (int it = 0; it < UI; ++it) {
+      const uint64_t y = x + (uint64_t)it * kWave + lane;
+      ok[it] = !TAIL || y < pend;
+      c[it] = ok[it] ? seg[y] : 0u;
+    }
+#pragma unroll
+    for (uint32_t b = 0; b < kRegCols; ++b) {
+      if (mode[b] == 0) {
+#pragma unroll
+        for (int it = 0; it < UI; ++it) ok[it] = ok[it] && hash_contains(hs, c[it]);
+      } else if (mode[b] == 1) {
+        uint32_t w[UI];
+#pragma unroll
+        for (int it = 0; it < UI; ++it) w[it] = tgt[b][c[it] >> 5];
+#pragma unroll
+        for (int it = 0; it < UI; ++it) ok[it] = ok[it] && ((w[it] >> (c[it] & 31)) & 1u);
+      } else if (mode[b] == 2) {
+#pragma unroll
+        for (int it = 0; it < UI; ++it) {
+          if (!ok[it]) continue;
+          const uint32_t pos = lower_bound_u32(tgt[b], tlen[b], c[it]);
+          ok[it] = pos < tlen[b] && tgt[b][pos] == c[it];
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kRegCols; ++k) {
+      if (k >= sc.n_ne) break;
+      const uint32_t m = pick3(sc.ne_col[k], k0, k1, k2);
+#pragma unroll
+      for (int it = 0; it < UI; ++it) ok[it] = ok[it] && c[it] != m;
+    }
+    uint64_t valid = 0;
+#pragma unroll
+    for (int it = 0; it < UI; ++it) {
+      if (!ok[it]) continue;
+      if (OP != OP_IPATH && a.other >= 0) {
+        const uint64_t r = cand_row(a, (uint32_t)a.other, q);
+        const uint64_t cb = a.cand_off[a.other][r];
+        const uint32_t nn = (uint32_t)(a.cand_off[a.other][r + 1] - cb);
+        acc += count_against(a.cand[a.other] + cb, nn, c[it], a.other_op);
+      } else {
+        valid += 1;
+      }
+    }
+    if (OP == OP_IPATH) {
+      const uint64_t n_valid = wave_sum(valid);
+      if (lane == 0 && n_valid) atomicAdd(path_counts + q, (unsigned long long)n_valid);
+    } else {
+      acc += valid;
+    }
+    if (TAIL) {
+      x = pend;
+      break;
+    }
+  }
+  return acc;
+}
 
 template <int OP, int NI, int NCOL>
 __device__ inline uint64_t walk_chunk_reg(const vsgm_level_args& a, const StageCols& sc, const uint64_t* __restrict__ seg_begin,
@@ -1586,7 +1700,22 @@ __device__ inline uint64_t walk_chunk_reg(const vsgm_level_args& a, const StageC
     const uint32_t w_c2 = (in && NCOL > 2) ? a.inst[sc.col[2]][q] : 0u;
     const uint64_t wend = min(e1, readlane_u64(w_end, n - 1));
     uint32_t ci = 0;  // window path containing the current block's first element
-    for (; x < wend; x += (uint64_t)NI * kWave) {
+    while (x < wend) {
+      // long stretch of one path: wave-uniform steps
+      while (readlane_u64(w_end, ci) <= x) ++ci;
+      {
+        const uint64_t pend = min(readlane_u64(w_end, ci), wend);
+        if (pend - x >= kUniTail) {
+          const uint64_t st = readlane_u64(w_start, ci), packed = readlane_u64(w_beg, ci);
+          const uint32_t k0 = NCOL > 0 ? __builtin_amdgcn_readlane(w_c0, ci) : 0u;
+          const uint32_t k1 = NCOL > 1 ? __builtin_amdgcn_readlane(w_c1, ci) : 0u;
+          const uint32_t k2 = NCOL > 2 ? __builtin_amdgcn_readlane(w_c2, ci) : 0u;
+          acc += uniform_steps<OP, kUniLarge, NCOL>(a, sc, p + ci, st, packed, k0, k1, k2, pend, x, hs, path_counts);
+          acc += uniform_steps<OP, kUniSmall, NCOL>(a, sc, p + ci, st, packed, k0, k1, k2, pend, x, hs, path_counts);
+          if (pend - x >= kUniTail) acc += uniform_steps<OP, kUniSmall, NCOL, true>(a, sc, p + ci, st, packed, k0, k1, k2, pend, x, hs, path_counts);
+          continue;
+        }
+      }
       uint32_t c[NI], piv[NI], c0[NI], c1[NI], c2[NI], pk[NI];
       bool ok[NI];
 #pragma unroll
@@ -1675,6 +1804,7 @@ __device__ inline uint64_t walk_chunk_reg(const vsgm_level_args& a, const StageC
           acc += 1;
         }
       }
+      x += (uint64_t)NI * kWave;
     }
     x = wend;
     p += n;

@@ -50,7 +50,7 @@ struct PlanTuning {
   uint32_t isect_kernel = 3;
   uint32_t max_grid = 8192;
   uint64_t batch_bytes = 0;     // 0: the executor's option
-  uint32_t cold_weight = 2;     // isect_kernel 3 pivot cost of a cold list test
+  uint32_t cold_weight = 1;     // isect_kernel 3 pivot cost of a cold list test
   uint32_t ticket_chunks = 64;  // isect_kernel 3 chunks per ticket
 };
 
