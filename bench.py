@@ -90,10 +90,10 @@ def intersection_rates(plan_text, ops, levels, seconds):
 def cpu_baseline(O, rp, cl, plan_texts, budget_s, threads, seed=12345):
     """Oracle (C restatement of the level algorithm, OpenMP over roots) on a
     seeded uniform sample of root vertices, all patterns. The sample grows
-    4x while the next pass is predicted to fit budget_s (a pass takes at
-    least budget_s / 8 unless every root is in); the rate of the last pass
-    is reported (embeddings found from the sampled roots per second),
-    together with its roots and per-pattern counts."""
+    4x until a pass takes at least budget_s / 4, every root is in, or the
+    next pass would not fit in what is left of budget_s; the rate of the
+    last pass is reported (embeddings found from the sampled roots per
+    second), together with its roots and per-pattern counts."""
     import numpy as np
 
     n = len(rp) - 1
@@ -109,7 +109,7 @@ def cpu_baseline(O, rp, cl, plan_texts, budget_s, threads, seed=12345):
         spent += dt
         total = sum(per)
         log(f"cpu baseline sample {len(roots)} roots: {total} embeddings in {dt:.2f}s")
-        if dt * 4 > budget_s - spent or dt >= budget_s / 8 or m >= n:
+        if dt * 4 > budget_s - spent or dt >= budget_s / 4 or m >= n:
             return {
                 "value": total / dt if dt > 0 else 0.0,
                 "unit": "embeddings/s",
