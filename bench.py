@@ -177,7 +177,7 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--patterns", default="kc4,p2,p1")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--detail-out", default="")
     ap.add_argument("--hub-bytes", type=float, default=-1, help="membership index bytes (-1 auto, 0 off)")
@@ -406,7 +406,10 @@ def make_roofline(kstats, workload, plan, isect_kernel):
            "kernel": dom["name"], "launches": dom["launches"], "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 3),
            "algo_bytes_per_launch": int(per_launch_bytes), "bound_note": note}
     if traffic:
+        # bytes the launches really moved across the fabric (rocprof FETCH_SIZE + WRITE_SIZE,
+        # calibrated), over the same launch time: how busy the memory system was
         out["traffic_gbs"] = round(traffic / per_launch_s / 1e9, 1)
+        out["traffic_frac"] = round(traffic / per_launch_s / 1e9 / PEAK_HBM_GBS, 4)
         out["traffic_profile"] = f"profiles/traffic_{dom['name']}_{workload}_{plan}_k{isect_kernel}.json"
     return out
 

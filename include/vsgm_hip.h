@@ -216,8 +216,9 @@ typedef struct vsgm_op_stats {
  * "<stage>_<op>" with stage setup / map / scatter and op icount (COMPUTE_COUNT),
  * iwrite (COMPUTE), expand (MATERIALIZE), filter (FILTER_COMPUTE), count2
  * (two-set COUNT), ipath (per-path counts). algo_bytes sums the launches'
- * algorithmic HBM bytes (for intersections the merge model: every backward
- * adjacency list read once, plus each path's bound vertices and row_ptrs). */
+ * algorithmic HBM bytes (for intersections the streaming model: every
+ * candidate element read once from its pivot list, plus each path's bound
+ * vertices, row_ptrs and segment words). */
 typedef struct vsgm_kernel_stat {
   char name[32];
   uint64_t launches;

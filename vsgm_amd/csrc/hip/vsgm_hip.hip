@@ -2336,16 +2336,20 @@ int run_setup_scan(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t**
   return VSGM_OK;
 }
 
-// Algorithmic bytes of the map kernel: intersections read every backward
-// adjacency list once (the merge model: 4 * S), each path's bound vertices
-// and row_ptrs, and its segment words; the other ops stream their W
-// candidate elements.
+// Algorithmic bytes of the map kernel (the streaming model): every one of
+// the W candidate elements is read once from its pivot list, and each path's
+// bound vertices, row_ptrs and segment words once. Membership tests against
+// the other lists are not counted: they are answered from the LDS table or
+// the membership index, whose traffic rocprof measures. (The merge model,
+// every backward list read whole per path, 4 * adj_entries, over-counts
+// what a probing intersection must move, so it is not used as the bound.)
 template <int OP>
 uint64_t map_bytes(const vsgm_ctx* ctx, const vsgm_level_args& a, uint64_t W, uint64_t n_chunks) {
+  (void)ctx;
   const uint64_t P = a.path_num;
   uint64_t b;
   if (is_intersect(OP)) {
-    b = 4 * ctx->stats.adj_entries + P * ((4 + 16) * (uint64_t)a.n_back + 16);
+    b = 4 * W + P * ((4 + 16) * (uint64_t)a.n_back + 16);
     if (OP == OP_ICOUNT && a.other >= 0) b += P * (8 + 16);
   } else if (OP == OP_COUNT2) {
     b = 4 * W + P * (2 * (8 + 16) + 16);
