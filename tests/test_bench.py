@@ -65,3 +65,24 @@ def test_budget_without_warm_up():
     c = _Clock(50.0)
     warm, timed, t0, t1 = bench.schedule_steps(c.step, c.step, c.restart, 0, 3, 480.0, c.now)
     assert (warm, timed) == (0, 3) and t1 - t0 == 150.0
+
+
+def test_roofline_uses_traffic_only_for_its_own_configuration(tmp_path, monkeypatch):
+    kstats = [{"name": "map_icount", "launches": 10, "total_ms": 100.0, "algo_bytes": 10 * 4_000_000_000}]
+    # 4e9 algorithmic bytes per 10 ms launch = 400 GB/s
+    r = bench.make_roofline(kstats, "rmat22", "heuristic", 3)
+    assert r["achieved"] == 400.0 and r["frac"] == 0.05 and r["bound"] == "hbm"
+    assert r["traffic"] is not None  # the committed profile matches this source and configuration
+    assert r["traffic_gbs"] == round(r["traffic"] / 0.010 / 1e9, 1)
+    # another workload, plan or map kernel: no traffic
+    for args in (("rmat20", "heuristic", 3), ("rmat22", "learned", 3), ("rmat22", "heuristic", 0)):
+        assert bench.make_roofline(kstats, *args)["traffic"] is None
+    # a profile measured on another kernel source is not used
+    monkeypatch.setattr(bench, "kernel_source_sha", lambda: "0000000000000000")
+    assert bench.make_roofline(kstats, "rmat22", "heuristic", 3)["traffic"] is None
+
+
+def test_roofline_frac_never_exceeds_one():
+    kstats = [{"name": "map_x", "launches": 1, "total_ms": 1.0, "algo_bytes": 20_000_000_000}]  # 20 TB/s "algorithmic"
+    r = bench.make_roofline(kstats, "nowhere", "heuristic", 3)
+    assert r["frac"] == 1.0 and "capped" in r["bound_note"]
