@@ -182,8 +182,8 @@ def main():
     ap.add_argument("--detail-out", default="")
     ap.add_argument("--hub-bytes", type=float, default=-1, help="membership index bytes (-1 auto, 0 off)")
     ap.add_argument("--hub-min-degree", type=int, default=64)
-    ap.add_argument("--isect-kernel", type=int, default=3)
-    ap.add_argument("--cold-weight", type=int, default=1)
+    ap.add_argument("--isect-kernel", type=int, default=4)
+    ap.add_argument("--cold-weight", type=int, default=16)
     ap.add_argument("--ticket-chunks", type=int, default=64)
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
     ap.add_argument("--cl-vertices", type=int, default=65_600_000)
@@ -633,7 +633,7 @@ def batched_main(args):
         tx.set_graph(g)
         tx.add_device_view(bin_sources[0], v_hop, v_rd)
         t = time.perf_counter()
-        res = autotune.tune(tx, repeats=1, candidates={"isect_kernel": [3, 0], "max_grid": [8192], "batch_bytes": [0, 1 << 30],
+        res = autotune.tune(tx, repeats=1, candidates={"isect_kernel": [4, 3, 0], "max_grid": [8192], "batch_bytes": [0, 1 << 30],
                                                        "ticket_chunks": [16, 64]})
         tune_s = time.perf_counter() - t
         del tx
@@ -654,7 +654,7 @@ def batched_main(args):
         bx.replay()
         run_s = time.perf_counter() - t
         counts = bx.total_counts
-        log(f"{label}: tuned in {tune_s:.1f}s ({sum(r['config'] != {'isect_kernel': 3, 'max_grid': 8192, 'batch_bytes': 0, 'cold_weight': 1, 'ticket_chunks': 64} for r in res.values())} "
+        log(f"{label}: tuned in {tune_s:.1f}s ({sum(r['config'] != {'isect_kernel': 4, 'max_grid': 8192, 'batch_bytes': 0, 'cold_weight': 16, 'ticket_chunks': 64} for r in res.values())} "
             f"plans off the default), captured {nodes} nodes in {cap_s:.1f}s, replay {run_s:.3f}s")
         return counts, run_s, nodes
 

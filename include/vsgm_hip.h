@@ -134,7 +134,12 @@ typedef struct vsgm_level_args {
   const uint32_t* hub_slot;  /* u32[V]: bitmap row of an indexed vertex, ~0u otherwise */
   const uint32_t* hub_bits;  /* u32[n_hubs * hub_words]: bit c of row s = c in adj(hub s) */
   uint64_t hub_words;        /* u32 words per bitmap row */
+  uint32_t flags;            /* VSGM_LEVEL_* properties of the view (0: none known) */
 } vsgm_level_args;
+/* The view has no self-loop, so a candidate drawn from or tested against
+ * the list of a bound vertex v can never equal v: '!=' conditions against
+ * backward vertices are then implied and the intersection maps skip them. */
+#define VSGM_LEVEL_LOOP_FREE 1u
 
 /* Dense membership index of the highest-degree lists of a view. No
  * counterpart in the reference, whose connectivity check always binary-
@@ -153,6 +158,9 @@ typedef struct vsgm_hub_index {
 int vsgm_hub_index_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, const uint32_t* h_hubs,
                          uint32_t n_hubs, vsgm_hub_index* out);
 int vsgm_hub_index_free(vsgm_ctx* ctx, vsgm_hub_index* index);
+/* Number of self-loops (v in adj(v)) of a device view, into *count (host).
+ * No counterpart in the reference; it decides VSGM_LEVEL_LOOP_FREE. */
+int vsgm_view_self_loops(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, uint64_t* count);
 
 /* View of one bin built on the device from the whole graph resident in
  * HBM (replaces the host ViewBin::Materialize + ExtendFromSource,
@@ -245,11 +253,19 @@ typedef struct vsgm_kernel_stat {
  *                    of ticket_chunks consecutive 8192-element chunks from a
  *                    device counter; the pivot choice then prefers streaming
  *                    the list that changes per path (other ops run as 0);
+ *                 4: as 3, and COMPUTE_COUNT levels with two or three
+ *                    backward lists, no second computed set and at most
+ *                    three '!=' conditions run the flat map: each wavefront
+ *                    walks its chunk in 1024-element blocks that may span
+ *                    paths, with every path's probe plan resolved when a
+ *                    window of 64 paths is loaded (lists up to 12800
+ *                    entries held, power-of-two cuckoo tables);
  *   max_grid      cap on workgroups per map launch (grid-stride beyond);
  *   cold_weight   pivot cost of one test against a list that changes per
  *                 path, in units of one streamed pivot element (isect_kernel
- *                 3; 1..1024);
- *   ticket_chunks chunks per ticket of isect_kernel 3 (1..64). */
+ *                 3 and 4; 1..1024);
+ *   ticket_chunks chunks per ticket of isect_kernel 3 and 4 (1..64).
+ * Defaults: isect_kernel 4, max_grid 8192, cold_weight 16, ticket_chunks 64. */
 typedef struct vsgm_tuning {
   uint32_t isect_kernel;
   uint32_t max_grid;

@@ -50,4 +50,6 @@ def test_level_args_layout_matches_header():
     # the ctypes mirror of vsgm_level_args must match the C layout
     from vsgm_amd.hip import LevelArgs
 
-    assert ctypes.sizeof(LevelArgs) == 8 * 4 + 8 * 7 * 5 + 4 * 2 + 4 * 7 + 4 + 8 * 7 + 4 + 4 + 8 * 3
+    # ... + flags (u32) and the tail padding to 8-byte alignment
+    assert ctypes.sizeof(LevelArgs) == 8 * 4 + 8 * 7 * 5 + 4 * 2 + 4 * 7 + 4 + 8 * 7 + 4 + 4 + 8 * 3 + 8
+    assert LevelArgs.flags.offset == 440

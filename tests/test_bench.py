@@ -1,5 +1,6 @@
 """CPU: bench.py's host-side pieces — the --gpus guard and the
 intersection rates it reports beside the metric."""
+import json
 import os
 import subprocess
 import sys
@@ -68,18 +69,23 @@ def test_budget_without_warm_up():
 
 
 def test_roofline_uses_traffic_only_for_its_own_configuration(tmp_path, monkeypatch):
+    # a traffic profile of map_icount on rmat22 / heuristic / kernel 4, measured on this kernel source
+    (tmp_path / "profiles").mkdir()
+    (tmp_path / "profiles" / "traffic_map_icount_rmat22_heuristic_k4.json").write_text(
+        json.dumps({"kernel_source_sha": bench.kernel_source_sha(), "bytes_per_launch": 2.0e11}))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
     kstats = [{"name": "map_icount", "launches": 10, "total_ms": 100.0, "algo_bytes": 10 * 4_000_000_000}]
     # 4e9 algorithmic bytes per 10 ms launch = 400 GB/s
-    r = bench.make_roofline(kstats, "rmat22", "heuristic", 3)
+    r = bench.make_roofline(kstats, "rmat22", "heuristic", 4)
     assert r["achieved"] == 400.0 and r["frac"] == 0.05 and r["bound"] == "hbm"
-    assert r["traffic"] is not None  # the committed profile matches this source and configuration
+    assert r["traffic"] == 2.0e11
     assert r["traffic_gbs"] == round(r["traffic"] / 0.010 / 1e9, 1)
     # another workload, plan or map kernel: no traffic
-    for args in (("rmat20", "heuristic", 3), ("rmat22", "learned", 3), ("rmat22", "heuristic", 0)):
+    for args in (("rmat20", "heuristic", 4), ("rmat22", "learned", 4), ("rmat22", "heuristic", 3)):
         assert bench.make_roofline(kstats, *args)["traffic"] is None
     # a profile measured on another kernel source is not used
     monkeypatch.setattr(bench, "kernel_source_sha", lambda: "0000000000000000")
-    assert bench.make_roofline(kstats, "rmat22", "heuristic", 3)["traffic"] is None
+    assert bench.make_roofline(kstats, "rmat22", "heuristic", 4)["traffic"] is None
 
 
 def test_roofline_frac_never_exceeds_one():

@@ -67,7 +67,7 @@ def test_plan_tuning_keeps_counts(vsgm, setup):
     ex = vsgm.PipelineExecutor(0, g.vertex_count, plans)
     for i in range(len(plans)):
         ex.set_plan_tuning(i, isect_kernel=i % 3, max_grid=(256, 2048, 8192)[i % 3], batch_bytes=(0, 1 << 16)[i % 2])
-    assert ex.plan_tuning(4) == {"isect_kernel": 1, "max_grid": 2048, "batch_bytes": 0, "cold_weight": 1, "ticket_chunks": 64}
+    assert ex.plan_tuning(4) == {"isect_kernel": 1, "max_grid": 2048, "batch_bytes": 0, "cold_weight": 16, "ticket_chunks": 64}
     ex.transfer_graph(g)
     print("[gpu] per-plan tuning", flush=True)
     ex.match()
@@ -111,7 +111,7 @@ def test_autotuner_installs_a_config_and_keeps_counts(vsgm, setup):
     g, plans, want = setup
     ex = vsgm.PipelineExecutor(0, g.vertex_count, plans)
     ex.transfer_graph(g)
-    cands = {"isect_kernel": [3, 0, 2], "max_grid": [1024, 8192], "batch_bytes": [0, 1 << 16], "ticket_chunks": [16, 64]}
+    cands = {"isect_kernel": [4, 0, 2], "max_grid": [1024, 8192], "batch_bytes": [0, 1 << 16], "ticket_chunks": [16, 64]}
     print("[gpu] autotune 3 plans", flush=True)
     res = autotune.tune(ex, plans=[0, len(plans) - 2, len(plans) - 1], repeats=1, candidates=cands)
     for p, r in res.items():

@@ -61,7 +61,7 @@ def test_noalloc_matches_the_allocating_count(gpu):
     small, want_small, keep_small = _level(ctx, 900, 8_000, 4)
     assert small.path_num < big.path_num
     tot = ctx.empty(1, np.uint64)
-    for kernel in (0, 3):
+    for kernel in (0, 3, 4):
         ctx.tuning(isect_kernel=kernel)
         print(f"[gpu] noalloc isect_kernel={kernel}: paths {big.path_num} then {small.path_num}", flush=True)
         assert _count(ctx, "vsgm_intersect_count_total", big, tot) == want_big

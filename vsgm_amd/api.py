@@ -537,7 +537,7 @@ class PipelineExecutor(_Handle):
         _check(lib().vsgm_executor_peak_bytes(self._h, C.byref(b)), "peak")
         return b.value
 
-    def set_plan_tuning(self, plan: int, isect_kernel: int = 3, max_grid: int = 8192, batch_bytes: int = 0, cold_weight: int = 1,
+    def set_plan_tuning(self, plan: int, isect_kernel: int = 4, max_grid: int = 8192, batch_bytes: int = 0, cold_weight: int = 16,
                         ticket_chunks: int = 64):
         _check(lib().vsgm_executor_set_plan_tuning(self._h, plan, isect_kernel, max_grid, batch_bytes, cold_weight, ticket_chunks),
                "plan tuning")

@@ -6,13 +6,15 @@ Knobs (per plan, applied by the executor before the plan's levels run):
                 1 wave-cursor LDS tiles (256-entry tile width per wavefront),
                 2 per-lane search with one LDS-staged list per workgroup
                 (4096-entry stage), 3 one workgroup per CU with the stable
-                list of a run of paths in an LDS hash table;
+                list of a run of paths in an LDS hash table, 4 as 3 with the
+                flat map (1024-element blocks across paths, software-
+                pipelined) for the COMPUTE_COUNT levels it covers;
   max_grid      workgroups per map launch (grid-stride beyond);
   batch_bytes   per-level batch budget (0: budget / remaining levels);
-  ticket_chunks work-chunk size of isect_kernel 3: consecutive 8192-element
+  ticket_chunks work-chunk size of isect_kernel 3 / 4: consecutive 8192-element
                 chunks a workgroup takes per ticket (its LDS table is rebuilt
                 at most once per run change within a ticket);
-  cold_weight   isect_kernel 3's pivot cost of a test against a list that
+  cold_weight   isect_kernel 3 / 4's pivot cost of a test against a list that
                 changes per path (streaming a pivot element costs 1).
 The element chunk of kernels 0-2 is fixed at 2048 (it sizes the LDS copy of
 the chunk's segment offsets).
@@ -28,11 +30,11 @@ import time
 from typing import Dict, List, Optional, Sequence
 
 CANDIDATES = {
-    "isect_kernel": [0, 1, 2, 3],
+    "isect_kernel": [0, 1, 2, 3, 4],
     "max_grid": [2048, 8192, 32768],
     "batch_bytes": [0, 1 << 28, 1 << 31],
     "ticket_chunks": [4, 16, 64],
-    "cold_weight": [4, 8, 16],
+    "cold_weight": [1, 4, 16, 64],
 }
 
 

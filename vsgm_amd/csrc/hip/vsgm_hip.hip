@@ -53,7 +53,7 @@ constexpr int kMaxGrid = 8192;
 // entries keep the load factor <= 0.45.
 constexpr uint32_t kCuckooSlots = 14336;
 constexpr uint32_t kHashCap = 12800;
-constexpr uint32_t kDefaultColdWeight = 1;
+constexpr uint32_t kDefaultColdWeight = 16;
 constexpr uint32_t kDefaultTicketChunks = 64;
 
 enum OpKind { OP_ICOUNT = 0, OP_IWRITE = 1, OP_EXPAND = 2, OP_FILTER = 3, OP_COUNT2 = 4, OP_IPATH = 5 };
@@ -403,6 +403,19 @@ __global__ __launch_bounds__(kBlock) void k_fill(uint64_t* d, uint64_t n, uint64
 // Hub index: slot[hubs[i]] = i (slot pre-filled with ~0u), then one
 // workgroup per hub sets the bits of its adjacency list. Lists are sorted,
 // so neighbouring lanes often hit the same word: atomicOr merges them.
+__global__ __launch_bounds__(kBlock) void k_self_loops(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ cols, uint64_t V,
+                                                      unsigned long long* __restrict__ count) {
+  uint64_t n = 0;
+  for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t rb = row_ptrs[v];
+    const uint32_t len = (uint32_t)(row_ptrs[v + 1] - rb);
+    const uint32_t pos = lower_bound_u32(cols + rb, len, (uint32_t)v);
+    n += (pos < len && cols[rb + pos] == (uint32_t)v) ? 1 : 0;
+  }
+  n = wave_sum(n);
+  if ((threadIdx.x & (kWave - 1)) == 0 && n) atomicAdd(count, (unsigned long long)n);
+}
+
 __global__ __launch_bounds__(kBlock) void k_hub_slots(const uint32_t* __restrict__ hubs, uint32_t n_hubs, uint32_t* __restrict__ slot) {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_hubs; i += gridDim.x * kBlock) slot[hubs[i]] = i;
 }
@@ -1892,6 +1905,415 @@ __global__ __launch_bounds__(kHThreads) void k_map_hash(vsgm_level_args a, Stage
   }
 }
 
+// ---------------------------------------------------------------------------
+// Flat hashed map (isect_kernel 4): COMPUTE_COUNT with no second computed
+// set and two or three backward lists.
+//
+// Same tickets, chunk plan and held list as k_map_hash, but a wavefront
+// walks its chunk in blocks of kFlatRows x 64 consecutive elements that may
+// span several paths:
+// * Each path's probe plan (per backward position: held in the LDS table,
+//   membership-index row, binary search, or the pivot itself) and its
+//   segment base are resolved when a window of 64 paths is loaded, one lane
+//   per path, so a path costs no dependent load of its own.
+// * A block inside one path reads everything uniform (scalar registers) and
+//   loads its candidates with immediate offsets from one address; a block
+//   over several paths finds each element's path by a uniform loop over the
+//   block's path boundaries and gathers the path data across lanes.
+// * The table hashes are 24-bit multiplicative (v_mul_u32_u24, full rate)
+//   into power-of-two tables; hits are counted by ballot and popcount.
+
+constexpr uint32_t kFlatSlots = 16384;  // slots per cuckoo table: two tables = 128 KiB of LDS
+constexpr int kFlatRows = 12;           // rows of 64 elements per block (two buffers in flight)
+constexpr uint64_t kFlatBlock = (uint64_t)kFlatRows * kWave;
+constexpr uint32_t kFlatK0 = 0x9E3779u;  // odd 24-bit multipliers
+constexpr uint32_t kFlatK1 = 0xC2B2AFu;
+constexpr uint32_t kModeHeld = 0, kModeIndex = 1, kModeSearch = 2, kModePivot = 3;
+
+// Slot of c in table 0 / 1 of 2^(32 - sh) slots. Table 0 hashes the low 24
+// bits of c, table 1 the low 24 bits of c rotated by 16; two keys that agree
+// in both are equal. Both operands of each product fit in 24 bits, so it
+// compiles to the full-rate v_mul_u32_u24 (the inputs are masked in plain
+// C: the 24-bit multiply builtins leave wider inputs unspecified, and the
+// build and the lookups must hash alike).
+__device__ inline uint32_t flat_h0(uint32_t c, uint32_t sh) { return ((c & 0xFFFFFFu) * kFlatK0) >> sh; }
+__device__ inline uint32_t flat_h1(uint32_t c, uint32_t sh) { return ((__funnelshift_r(c, c, 16) & 0xFFFFFFu) * kFlatK1) >> sh; }
+
+struct FlatTable {
+  const uint32_t* t0;
+  const uint32_t* t1;
+  uint32_t sh;
+  uint32_t v;  // vertex whose list the tables hold (~0u: none)
+};
+
+__device__ inline bool flat_contains(const FlatTable& h, uint32_t c) {
+  return h.t0[flat_h0(c, h.sh)] == c || h.t1[flat_h1(c, h.sh)] == c;
+}
+
+// Cuckoo build of the n entries of list into t0 / t1 of 2^(32 - sh) slots
+// each, by every thread of the workgroup (barriers on both sides); false
+// when an insertion is still evicting after kCuckooRounds steps.
+__device__ inline bool flat_build(uint32_t* t0, uint32_t* t1, uint32_t sh, const uint32_t* __restrict__ list, uint32_t n, uint32_t* fail) {
+  const uint32_t m = 1u << (32 - sh);
+  for (uint32_t i = threadIdx.x; i < m; i += kHThreads) {
+    t0[i] = kEmptySlot;
+    t1[i] = kEmptySlot;
+  }
+  if (threadIdx.x == 0) *fail = 0;
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < n; j += kHThreads) {
+    uint32_t x = list[j];
+    uint32_t t = 0;
+    bool placed = false;
+    for (int r = 0; r < kCuckooRounds && !placed; ++r) {
+      uint32_t* slot = t == 0 ? &t0[flat_h0(x, sh)] : &t1[flat_h1(x, sh)];
+      x = atomicExch(slot, x);
+      placed = x == kEmptySlot;
+      t ^= 1u;
+    }
+    if (!placed) *fail = 1;
+  }
+  __syncthreads();
+  return *fail == 0;
+}
+
+// NE operands of a level that the flat map checks (pattern vertices), and
+// the backward vertices.
+struct FlatCols {
+  uint32_t n_ne;
+  uint32_t ne[VSGM_MAX_QV];
+};
+
+__device__ inline uint64_t shfl_u64(uint64_t v, uint32_t l) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)l);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Membership of c in a tested list under one probe mode (kModeHeld: the
+// table; kModeIndex: bit c of the index row at word offset t; kModeSearch:
+// binary search of the n entries at cols + t).
+__device__ inline bool flat_test(const vsgm_level_args& a, const FlatTable& h, uint32_t mode, uint64_t t, uint32_t n, uint32_t c) {
+  if (mode == kModeHeld) return flat_contains(h, c);
+  if (mode == kModeIndex) return (a.hub_bits[t + (c >> 5)] >> (c & 31)) & 1u;
+  const uint32_t pos = lower_bound_u32(a.cols + t, n, c);
+  return pos < n && a.cols[t + pos] == c;
+}
+
+// Rows of candidates: c[it] is the candidate of element it * 64 + lane of
+// the block, or kNoCand where there is none or a test has failed (no vertex
+// id equals kNoCand: views have fewer than kNoCand vertices).
+constexpr uint32_t kNoCand = 0xFFFFFFFEu;
+
+// Tests of the rows c[] against one backward position under a probe plan
+// that is the same for every element of the block; failing rows become
+// kNoCand.
+template <int R>
+__device__ inline void flat_rows_test(const vsgm_level_args& a, const FlatTable& h, uint32_t mode, uint64_t t, uint32_t n, uint32_t* c) {
+  if (mode == kModeHeld) {
+#pragma unroll
+    for (int it = 0; it < R; ++it) c[it] = flat_contains(h, c[it]) ? c[it] : kNoCand;
+  } else if (mode == kModeIndex) {
+    const uint32_t* row = a.hub_bits + t;
+    uint32_t w[R];
+#pragma unroll
+    for (int it = 0; it < R; ++it) w[it] = c[it] != kNoCand ? row[c[it] >> 5] : 0u;
+#pragma unroll
+    for (int it = 0; it < R; ++it) c[it] = ((w[it] >> (c[it] & 31)) & 1u) ? c[it] : kNoCand;
+  } else if (mode == kModeSearch) {
+    // one list for all rows: binary lifting, every row's probe of a step in flight together
+    const uint32_t* L = a.cols + t;
+    uint32_t pos[R];
+#pragma unroll
+    for (int it = 0; it < R; ++it) pos[it] = 0;
+    for (uint32_t step = top_step(n); step; step >>= 1) {
+#pragma unroll
+      for (int it = 0; it < R; ++it) {
+        const uint32_t probe = pos[it] + step;
+        const bool in = probe <= n;
+        const uint32_t v = L[in ? probe - 1 : 0];
+        pos[it] = (in && v < c[it]) ? probe : pos[it];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < R; ++it) c[it] = (pos[it] < n && L[pos[it]] == c[it]) ? c[it] : kNoCand;
+  }
+}
+
+// A block of one probe plan whose candidate loads are in flight (every
+// field wave-uniform).
+template <int NB, int NNE>
+struct FlatBlock {
+  uint32_t md;
+  uint64_t t[NB];
+  uint32_t nn[NB];
+  uint32_t ne[NNE > 0 ? NNE : 1];
+};
+
+// Tests and count of a block whose candidates c[] have been loaded.
+template <int NB, int NNE>
+__device__ inline uint64_t flat_finish(const vsgm_level_args& a, const FlatTable& h, const FlatBlock<NB, NNE>& bk, uint32_t* c) {
+#pragma unroll
+  for (int k = 0; k < NNE; ++k)
+#pragma unroll
+    for (int it = 0; it < kFlatRows; ++it) c[it] = c[it] != bk.ne[k] ? c[it] : kNoCand;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) flat_rows_test<kFlatRows>(a, h, (bk.md >> (2 * b)) & 3u, bk.t[b], bk.nn[b], c);
+  uint64_t n = 0;
+#pragma unroll
+  for (int it = 0; it < kFlatRows; ++it) n += (uint64_t)__popcll(__ballot(c[it] != kNoCand));
+  return n;
+}
+
+// Path window of a wavefront: lane l describes path p + l (l < n).
+template <int NB, int NNE>
+struct FlatWindow {
+  uint32_t n;
+  uint64_t st, en, base;  // segment start / end (elements), cols index of element e: base + e
+  uint32_t md;            // probe mode of each backward position (2 bits each)
+  uint64_t t[NB];
+  uint32_t nn[NB];
+  uint32_t ne[NNE > 0 ? NNE : 1];
+};
+
+// One block [x, bend) of the window: with one probe plan and uniform '!='
+// operands its loads are issued into c and its plan written to bk (returns
+// true); otherwise it is walked row by row with each element's own plan and
+// its count added to acc (returns false).
+template <int NB, int NNE>
+__device__ inline bool flat_block(const vsgm_level_args& a, const FlatTable& h, const FlatWindow<NB, NNE>& w, uint64_t x, uint64_t bend,
+                                  uint32_t* c, FlatBlock<NB, NNE>& bk, uint64_t& acc) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t lim = (uint32_t)(bend - x);
+  const uint32_t ci = (uint32_t)__popcll(__ballot(w.en <= x));
+  const uint32_t cj = (uint32_t)__popcll(__ballot(w.en < bend));
+  const bool span = lane >= ci && lane <= cj;
+  bk.md = __builtin_amdgcn_readlane(w.md, ci);
+  bool same = w.md == bk.md;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    bk.t[b] = readlane_u64(w.t[b], ci);
+    bk.nn[b] = __builtin_amdgcn_readlane(w.nn[b], ci);
+    same = same && w.t[b] == bk.t[b] && w.nn[b] == bk.nn[b];
+  }
+#pragma unroll
+  for (int k = 0; k < NNE; ++k) {
+    bk.ne[k] = __builtin_amdgcn_readlane(w.ne[k], ci);
+    same = same && w.ne[k] == bk.ne[k];
+  }
+  if (__ballot(span && !same) == 0) {
+    if (ci == cj) {
+      const uint32_t* src = a.cols + readlane_u64(w.base, ci) + x;
+#pragma unroll
+      for (int it = 0; it < kFlatRows; ++it) {
+        const uint32_t r = (uint32_t)it * kWave + lane;
+        c[it] = r < lim ? src[r] : kNoCand;
+      }
+    } else {
+      // element r of the block lies in path ci + #{path starts in (x, x + r]}
+      uint32_t pi[kFlatRows];
+#pragma unroll
+      for (int it = 0; it < kFlatRows; ++it) pi[it] = ci;
+      for (uint32_t l = ci + 1; l <= cj; ++l) {
+        const uint32_t s = (uint32_t)(readlane_u64(w.st, l) - x);
+#pragma unroll
+        for (int it = 0; it < kFlatRows; ++it) pi[it] += ((uint32_t)it * kWave + lane >= s) ? 1u : 0u;
+      }
+#pragma unroll
+      for (int it = 0; it < kFlatRows; ++it) {
+        const uint32_t r = (uint32_t)it * kWave + lane;
+        const uint64_t bse = shfl_u64(w.base, pi[it]);  // gathered with every lane active
+        c[it] = r < lim ? a.cols[bse + x + r] : kNoCand;
+      }
+    }
+    return true;
+  }
+  // paths with different probe plans: row by row, each element with its own path's plan
+#pragma unroll 1
+  for (uint32_t r0 = 0; r0 < lim; r0 += kWave) {
+    const uint32_t r = r0 + lane;
+    uint32_t pi = ci;
+    for (uint32_t l = ci + 1; l <= cj; ++l) pi += r >= (uint32_t)(readlane_u64(w.st, l) - x) ? 1u : 0u;
+    bool ok = r < lim;
+    const uint64_t bse = shfl_u64(w.base, pi);  // every lane active
+    const uint32_t cv = ok ? a.cols[bse + x + r] : 0u;
+    const uint32_t mdi = (uint32_t)__shfl((int)w.md, (int)pi);
+#pragma unroll
+    for (int k = 0; k < NNE; ++k) {
+      const uint32_t nv = (uint32_t)__shfl((int)w.ne[k], (int)pi);  // every lane active
+      ok = ok && cv != nv;
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const uint32_t mode = (mdi >> (2 * b)) & 3u;
+      const uint64_t tb = shfl_u64(w.t[b], pi);
+      const uint32_t nb = (uint32_t)__shfl((int)w.nn[b], (int)pi);
+      if (ok && mode != kModePivot) ok = flat_test(a, h, mode, tb, nb, cv);
+    }
+    acc += (uint64_t)__popcll(__ballot(ok));
+  }
+  return false;
+}
+
+// One wavefront's walk over chunk [e0, e1), paths first..last. Returns the
+// wave's count (uniform across the wave). Blocks are software-pipelined
+// over two register buffers: a block's candidate loads are issued before
+// the previous block's tests run.
+template <int NB, int NNE>
+__device__ inline uint64_t flat_walk(const vsgm_level_args& a, const FlatCols& fc, const uint64_t* __restrict__ seg_begin,
+                                     const uint64_t* __restrict__ seg_off, uint64_t e0, uint64_t e1, uint64_t first, uint64_t last,
+                                     const FlatTable& h) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  uint64_t acc = 0;
+  uint64_t p = first;
+  uint64_t x = e0;
+  uint32_t cA[kFlatRows], cB[kFlatRows];
+  FlatBlock<NB, NNE> bA, bB;
+  bool pA = false, pB = false;  // block whose loads are in flight in cA / cB
+  while (x < e1 && p <= last) {
+    FlatWindow<NB, NNE> w;
+    w.n = (uint32_t)min<uint64_t>(kWave, last - p + 1);
+    const bool in = lane < w.n;
+    const uint64_t q = p + lane;
+    w.st = in ? seg_off[q] : ~0ull;
+    w.en = in ? seg_off[q + 1] : ~0ull;
+    const uint64_t sb = in ? seg_begin[q] : 0;
+    w.base = (sb & kBeginMask) - w.st;
+    const uint32_t piv = (uint32_t)(sb >> 56);
+    uint32_t v[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) v[b] = in ? a.inst[a.back[b]][q] : 0u;
+#pragma unroll
+    for (int k = 0; k < NNE; ++k) w.ne[k] = in ? a.inst[fc.ne[k]][q] : 0u;
+    uint32_t sl[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const bool tested = in && (uint32_t)b != piv && v[b] != h.v;
+      sl[b] = (tested && a.hub_slot) ? a.hub_slot[v[b]] : ~0u;
+    }
+    w.md = 0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      uint32_t mode;
+      w.t[b] = 0;
+      w.nn[b] = 0;
+      if ((uint32_t)b == piv || !in) {
+        mode = kModePivot;
+      } else if (v[b] == h.v) {
+        mode = kModeHeld;
+      } else if (sl[b] != ~0u) {
+        mode = kModeIndex;
+        w.t[b] = (uint64_t)sl[b] * a.hub_words;
+      } else {
+        mode = kModeSearch;
+        const uint64_t rb = a.row_ptrs[v[b]];
+        w.t[b] = rb;
+        w.nn[b] = (uint32_t)(a.row_ptrs[v[b] + 1] - rb);
+      }
+      w.md |= mode << (2 * b);
+    }
+    const uint64_t wend = min(e1, readlane_u64(w.en, w.n - 1));
+    while (x < wend) {
+      {
+        const uint64_t bend = min(x + kFlatBlock, wend);
+        const bool fa = flat_block<NB, NNE>(a, h, w, x, bend, cA, bA, acc);
+        if (pB) acc += flat_finish<NB, NNE>(a, h, bB, cB);
+        pB = false;
+        pA = fa;
+        x = bend;
+      }
+      if (x >= wend) break;
+      {
+        const uint64_t bend = min(x + kFlatBlock, wend);
+        const bool fb = flat_block<NB, NNE>(a, h, w, x, bend, cB, bB, acc);
+        if (pA) acc += flat_finish<NB, NNE>(a, h, bA, cA);
+        pA = false;
+        pB = fb;
+        x = bend;
+      }
+    }
+    // the next window starts with buffer A again: nothing may stay in flight
+    if (pA) acc += flat_finish<NB, NNE>(a, h, bA, cA);
+    if (pB) acc += flat_finish<NB, NNE>(a, h, bB, cB);
+    pA = pB = false;
+    x = wend;
+    p += w.n;
+  }
+  return acc;
+}
+
+template <int NB, int NNE>
+__global__ __launch_bounds__(kHThreads) void k_map_flat(vsgm_level_args a, FlatCols fc, const uint64_t* __restrict__ seg_begin,
+                                                        const uint64_t* __restrict__ seg_off, uint64_t W, uint64_t c_lo, uint64_t c_hi,
+                                                        const ChunkPlan* __restrict__ plan, uint32_t ticket_chunks,
+                                                        unsigned long long* __restrict__ tickets, unsigned long long* __restrict__ total) {
+  __shared__ uint32_t s_cuckoo[2 * kFlatSlots];
+  __shared__ uint32_t s_fail;
+  __shared__ ChunkPlan s_plan[kTicketCap];
+  __shared__ uint32_t s_votes[kTicketCap];
+  __shared__ uint64_t s_tmp[kHWaves];
+  __shared__ uint64_t s_ticket;
+  __shared__ uint32_t s_next;
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t n_tickets = (c_hi - c_lo + ticket_chunks - 1) / ticket_chunks;
+  uint32_t hv = ~0u, hsh = 26;  // held vertex, hash shift (64 slots)
+  uint64_t acc = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_ticket = atomicAdd(tickets, 1ull);
+      s_next = 0;
+    }
+    __syncthreads();
+    const uint64_t t = readfirstlane_u64(s_ticket);
+    if (t >= n_tickets) break;
+    const uint64_t k_lo = c_lo + t * ticket_chunks, k_hi = min(k_lo + ticket_chunks, c_hi);
+    const uint32_t nk = (uint32_t)(k_hi - k_lo);
+    if (threadIdx.x < nk) s_plan[threadIdx.x] = plan[k_lo - c_lo + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < nk) {
+      const uint32_t v = s_plan[threadIdx.x].want_v;
+      uint32_t votes = 0;
+      if (v != ~0u)
+        for (uint32_t i = 0; i < nk; ++i) votes += s_plan[i].want_v == v ? 1u : 0u;
+      s_votes[threadIdx.x] = votes;
+    }
+    __syncthreads();
+    uint32_t best = 0;
+    for (uint32_t i = 1; i < nk; ++i)
+      if (s_votes[i] > s_votes[best]) best = i;
+    const uint32_t want = __builtin_amdgcn_readfirstlane(s_votes[best] ? s_plan[best].want_v : ~0u);
+    if (want != ~0u && want != hv) {
+      const uint32_t n = s_plan[best].want_n;
+      // smallest power of two >= 1.125 n + 64 slots per table (load <= 0.45)
+      const uint32_t need = n + n / 8 + 64;
+      uint32_t bits = 6;
+      while ((1u << bits) < need) ++bits;
+      hsh = 32 - bits;
+      hv = flat_build(s_cuckoo, s_cuckoo + kFlatSlots, hsh, a.cols + a.row_ptrs[want], n, &s_fail) ? want : ~0u;
+    }
+    const FlatTable h{s_cuckoo, s_cuckoo + kFlatSlots, hsh, hv};
+    for (;;) {
+      uint32_t j = 0;
+      if (lane == 0) j = atomicAdd(&s_next, 1u);
+      j = __builtin_amdgcn_readfirstlane(j);  // uniform (scalar) from here on
+      if (j >= nk) break;
+      const uint64_t first = readfirstlane_u64(s_plan[j].first), last = readfirstlane_u64(s_plan[j].last);
+      const uint64_t k = k_lo + j;
+      const uint64_t e0 = k * kHChunk, e1 = min(e0 + kHChunk, W);
+      acc += flat_walk<NB, NNE>(a, fc, seg_begin, seg_off, e0, e1, first, last, h);
+    }
+    __syncthreads();
+  }
+  if (lane == 0) s_tmp[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t sum = 0;
+    for (int w = 0; w < kHWaves; ++w) sum += s_tmp[w];
+    if (sum) atomicAdd(total, (unsigned long long)sum);
+  }
+}
+
 // Scatter the valid elements of the compacting ops; chunk_off is the
 // exclusive scan of the chunk counts (n_chunks + 1 entries).
 template <int OP>
@@ -2051,7 +2473,7 @@ struct vsgm_ctx {
   bool profiling = false;
   std::vector<KernelRecord> records;
   std::vector<hipEvent_t> spare_events;
-  vsgm_tuning tuning{3, 8192, kDefaultColdWeight, kDefaultTicketChunks};
+  vsgm_tuning tuning{4, 8192, kDefaultColdWeight, kDefaultTicketChunks};
   // VSGM_PROGRESS=<seconds>: long map launches report their progress on
   // stderr at most this often (0: silent)
   double progress_s = 0;
@@ -2321,7 +2743,7 @@ int run_setup_scan(vsgm_ctx* ctx, const vsgm_level_args& a, Temps& t, uint64_t**
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stats, 2, 0);
   {
     ProfScope ps(ctx, kname<OP>(0), P * setup_bytes_per_path<OP>(a), P, P);
-    const bool hashed = (OP == OP_ICOUNT || OP == OP_IPATH) && ctx->tuning.isect_kernel == 3 && a.n_back > 1;
+    const bool hashed = (OP == OP_ICOUNT || OP == OP_IPATH) && ctx->tuning.isect_kernel >= 3 && a.n_back > 1;
     hipLaunchKernelGGL(k_setup<OP>, dim3(grid_for((P + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, a, *seg_begin, *seg_off,
                        (unsigned long long*)(ctx->d_stats + 1), hashed ? kHashCap : 0u, ctx->tuning.cold_weight);
   }
@@ -2441,12 +2863,70 @@ int launch_map_hash(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg
   return launch_check(ctx, kname<OP>(1));
 }
 
+// Is 'candidate != vertex bound to u' implied? On a view without self-loops
+// it is whenever u is a backward vertex: the candidate lies in u's list.
+bool ne_implied(const vsgm_level_args& a, uint32_t u) {
+  if (!(a.flags & VSGM_LEVEL_LOOP_FREE)) return false;
+  for (uint32_t i = 0; i < a.n_back; ++i)
+    if (a.back[i] == u) return true;
+  return false;
+}
+
+// Levels the flat map runs: COMPUTE_COUNT with two or three backward lists,
+// no second computed set, at most three '!=' conditions.
+bool flat_eligible(const vsgm_level_args& a) {
+  if (a.other >= 0 || a.n_back < 2 || a.n_back > 3 || a.vertex_count >= kNoCand) return false;
+  uint32_t n_ne = 0;
+  for (uint32_t i = 0; i < a.n_cond; ++i) n_ne += (a.cond[i].op == VSGM_NON_EQUAL && !ne_implied(a, a.cond[i].operand)) ? 1u : 0u;
+  return n_ne <= 3;
+}
+
+// The flat map over chunks [c_lo, c_hi) (2048-element units): the chunk plan
+// pass, then k_map_flat on one workgroup per CU pulling tickets.
+int launch_map_flat(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t c_lo,
+                    uint64_t c_hi, ChunkPlan* plan, uint64_t bytes, uint64_t paths, unsigned long long* total) {
+  constexpr uint64_t kRatio = kHChunk / kChunk;
+  const uint64_t h_lo = c_lo / kRatio, h_hi = (c_hi + kRatio - 1) / kRatio;
+  const uint64_t e_lo = c_lo * kChunk, e_hi = std::min(W, c_hi * kChunk);
+  hipLaunchKernelGGL(k_chunk_plan, dim3(grid_for((h_hi - h_lo + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, a, seg_begin, seg_off, W,
+                     h_lo, h_hi, plan);
+  VSGM_TRY(launch_check(ctx, "k_chunk_plan"));
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stats + 2, 1, 0);
+  VSGM_TRY(launch_check(ctx, "k_fill"));
+  FlatCols fc{};
+  for (uint32_t i = 0; i < a.n_cond; ++i)
+    if (a.cond[i].op == VSGM_NON_EQUAL && !ne_implied(a, a.cond[i].operand)) fc.ne[fc.n_ne++] = a.cond[i].operand;
+  const uint32_t tc = ctx->tuning.ticket_chunks;
+  const uint64_t n_tickets = (h_hi - h_lo + tc - 1) / tc;
+  const unsigned grid = (unsigned)std::min<uint64_t>(std::min<uint64_t>(n_tickets, (uint64_t)ctx->n_cu), ctx->tuning.max_grid);
+  {
+    ProfScope ps(ctx, kname<OP_ICOUNT>(1), bytes, e_hi - e_lo, paths);
+#define VSGM_LAUNCH_FLAT(NB, NNE)                                                                                                        \
+  hipLaunchKernelGGL((k_map_flat<NB, NNE>), dim3(grid), dim3(kHThreads), 0, ctx->stream, a, fc, seg_begin, seg_off, W, h_lo, h_hi, plan, tc, \
+                     (unsigned long long*)(ctx->d_stats + 2), total)
+    const uint32_t key = a.n_back * 4 + fc.n_ne;
+    switch (key) {
+      case 8: VSGM_LAUNCH_FLAT(2, 0); break;
+      case 9: VSGM_LAUNCH_FLAT(2, 1); break;
+      case 10: VSGM_LAUNCH_FLAT(2, 2); break;
+      case 11: VSGM_LAUNCH_FLAT(2, 3); break;
+      case 12: VSGM_LAUNCH_FLAT(3, 0); break;
+      case 13: VSGM_LAUNCH_FLAT(3, 1); break;
+      case 14: VSGM_LAUNCH_FLAT(3, 2); break;
+      default: VSGM_LAUNCH_FLAT(3, 3); break;
+    }
+#undef VSGM_LAUNCH_FLAT
+  }
+  return launch_check(ctx, kname<OP_ICOUNT>(1));
+}
+
 template <int OP>
 int launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begin, const uint64_t* seg_off, uint64_t W, uint64_t n_chunks,
                uint64_t bytes, uint64_t* mask, uint64_t* chunk_cnt, unsigned long long* total, unsigned long long* path_counts) {
   const uint64_t n_windows = (n_chunks + kWindowChunks - 1) / kWindowChunks;
   constexpr bool kHashable = OP == OP_ICOUNT || OP == OP_IPATH;
-  const bool hashed = kHashable && ctx->tuning.isect_kernel == 3 && a.n_back > 1;
+  const bool hashed = kHashable && ctx->tuning.isect_kernel >= 3 && a.n_back > 1;
+  const bool flat = OP == OP_ICOUNT && ctx->tuning.isect_kernel == 4 && flat_eligible(a);
   Temps tmp(ctx);
   ChunkPlan* plan = nullptr;
   if (hashed) VSGM_TRY(tmp.get((std::min(n_chunks, kWindowChunks) * kChunk + kHChunk - 1) / kHChunk, &plan));
@@ -2459,7 +2939,10 @@ int launch_map(vsgm_ctx* ctx, const vsgm_level_args& a, const uint64_t* seg_begi
     const uint64_t share = (uint64_t)((unsigned __int128)bytes * (c_hi - c_lo) / n_chunks);
     bool done = false;
     if constexpr (kHashable) {
-      if (hashed) {
+      if (flat) {
+        VSGM_TRY(launch_map_flat(ctx, a, seg_begin, seg_off, W, c_lo, c_hi, plan, share, w == 0 ? a.path_num : 0, total));
+        done = true;
+      } else if (hashed) {
         VSGM_TRY(launch_map_hash<OP>(ctx, a, seg_begin, seg_off, W, c_lo, c_hi, plan, share, w == 0 ? a.path_num : 0, total, path_counts));
         done = true;
       }
@@ -2968,6 +3451,19 @@ int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d
   return VSGM_OK;
 }
 
+int vsgm_view_self_loops(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d_cols, uint64_t vertex_count, uint64_t* count) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (!count || (vertex_count && (!d_row_ptrs || !d_cols))) return fail(ctx, VSGM_ERR_ARG, "null view or output");
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stats, 1, 0);
+  VSGM_TRY(launch_check(ctx, "k_fill"));
+  if (vertex_count) {
+    hipLaunchKernelGGL(k_self_loops, dim3(grid_for((vertex_count + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols,
+                       vertex_count, (unsigned long long*)ctx->d_stats);
+    VSGM_TRY(launch_check(ctx, "k_self_loops"));
+  }
+  return read_u64(ctx, ctx->d_stats, count);
+}
+
 int vsgm_hub_index_free(vsgm_ctx* ctx, vsgm_hub_index* index) {
   if (!ctx || !index) return VSGM_ERR_ARG;
   VSGM_TRY(pool_free(ctx, index->slot));
@@ -3151,7 +3647,7 @@ int vsgm_split_batch(vsgm_ctx* ctx, const uint64_t* d_children, uint64_t n, uint
 
 int vsgm_set_tuning(vsgm_ctx* ctx, const vsgm_tuning* t) {
   if (!ctx || !t) return VSGM_ERR_ARG;
-  if (t->isect_kernel > 3 || t->max_grid == 0 || t->max_grid > (1u << 20)) return fail(ctx, VSGM_ERR_ARG, "bad tuning");
+  if (t->isect_kernel > 4 || t->max_grid == 0 || t->max_grid > (1u << 20)) return fail(ctx, VSGM_ERR_ARG, "bad tuning");
   if (t->cold_weight == 0 || t->cold_weight > 1024 || t->ticket_chunks == 0 || t->ticket_chunks > (uint32_t)kTicketCap)
     return fail(ctx, VSGM_ERR_ARG, "bad tuning: cold_weight in [1, 1024], ticket_chunks in [1, %d]", kTicketCap);
   ctx->tuning = *t;

@@ -480,7 +480,7 @@ int vsgm_executor_set_plan_tuning(vsgm_executor* e, int plan, uint32_t isect_ker
   return Guard([&] {
     VSGM_CHECK(e, "null argument");
     VSGM_CHECK(plan >= 0 && (size_t)plan < e->e->GetTotalCounts().size(), "plan index out of range");
-    if (isect_kernel > 3 || max_grid == 0 || max_grid > (1u << 20)) throw vsgm::Error("bad plan tuning");
+    if (isect_kernel > 4 || max_grid == 0 || max_grid > (1u << 20)) throw vsgm::Error("bad plan tuning");
     if (cold_weight == 0 || cold_weight > 1024 || ticket_chunks == 0 || ticket_chunks > 64) throw vsgm::Error("bad plan tuning");
     e->e->SetPlanTuning((size_t)plan, vsgm::PlanTuning{isect_kernel, max_grid, batch_bytes, cold_weight, ticket_chunks});
   });

@@ -124,7 +124,7 @@ void PipelineExecutor::SetPlans(std::vector<const Plan*> plans) {
 
 void PipelineExecutor::SetPlanTuning(size_t plan, const PlanTuning& t) {
   VSGM_CHECK(plan < plans_.size(), "plan index out of range");
-  VSGM_CHECK(t.isect_kernel <= 3 && t.max_grid > 0 && t.max_grid <= (1u << 20), "bad plan tuning");
+  VSGM_CHECK(t.isect_kernel <= 4 && t.max_grid > 0 && t.max_grid <= (1u << 20), "bad plan tuning");
   VSGM_CHECK(t.cold_weight >= 1 && t.cold_weight <= 1024 && t.ticket_chunks >= 1 && t.ticket_chunks <= 64, "bad plan tuning");
   ReleaseGraph();
   tuning_[plan] = t;
@@ -191,6 +191,9 @@ PipelineExecutor::View PipelineExecutor::UploadView(const uint64_t* row_ptrs, co
 }
 
 void PipelineExecutor::IndexView(View& v, const uint64_t* row_ptrs, uint64_t hub_limit) {
+  uint64_t loops = 0;
+  Check(vsgm_view_self_loops(ctx_, (const uint64_t*)v.row_ptrs->get(), (const uint32_t*)v.cols->get(), vertex_count_, &loops), "self-loop check");
+  v.loop_free = loops == 0;
   const uint64_t row_bytes = (vertex_count_ + 63) / 64 * 8;
   const uint64_t cap = std::min<uint64_t>(hub_limit / row_bytes, 0xFFFFFFFEull);
   if (cap == 0 || vertex_count_ == 0) return;
@@ -427,6 +430,7 @@ void PipelineExecutor::FillArgs(size_t level, const Level& l, vsgm_level_args& a
   a.hub_slot = view_->hub.slot;
   a.hub_bits = view_->hub.bits;
   a.hub_words = view_->hub.words;
+  a.flags = view_->loop_free ? VSGM_LEVEL_LOOP_FREE : 0u;
   for (int u = 0; u < kMaxQueryVerticesNum; ++u) {
     a.inst[u] = l.inst[u].ptr;
     a.cand[u] = l.cand[u].ptr;

@@ -47,11 +47,11 @@ class DeviceBuffer;
 
 // Launch-shape and budget choices for one plan (set by the autotuner).
 struct PlanTuning {
-  uint32_t isect_kernel = 3;
+  uint32_t isect_kernel = 4;
   uint32_t max_grid = 8192;
   uint64_t batch_bytes = 0;     // 0: the executor's option
-  uint32_t cold_weight = 1;     // isect_kernel 3 pivot cost of a cold list test
-  uint32_t ticket_chunks = 64;  // isect_kernel 3 chunks per ticket
+  uint32_t cold_weight = 16;    // isect_kernel 3 / 4 pivot cost of a cold list test
+  uint32_t ticket_chunks = 64;  // isect_kernel 3 / 4 chunks per ticket
 };
 
 class PipelineExecutor {
@@ -114,6 +114,7 @@ class PipelineExecutor {
     std::shared_ptr<DeviceBuffer> row_ptrs, cols, sources;
     uint64_t sources_num = 0;
     vsgm_hub_index hub{nullptr, nullptr, 0, 0};
+    bool loop_free = false;  // no v in adj(v): '!=' against backward vertices is implied
   };
   void ClearViews();
   void IndexView(View& v, const uint64_t* h_row_ptrs, uint64_t hub_limit);

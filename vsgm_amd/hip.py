@@ -53,6 +53,7 @@ class LevelArgs(C.Structure):
         ("hub_slot", C.c_void_p),
         ("hub_bits", C.c_void_p),
         ("hub_words", C.c_uint64),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -110,6 +111,7 @@ def _declare(L):
         "vsgm_last_op_stats": ([P, C.POINTER(OpStats)], i),
         "vsgm_hub_index_build": ([P, P, P, u64, P, C.c_uint32, C.POINTER(HubIndex)], i),
         "vsgm_hub_index_free": ([P, C.POINTER(HubIndex)], i),
+        "vsgm_view_self_loops": ([P, P, P, u64, C.POINTER(C.c_uint64)], i),
         "vsgm_csr_from_pairs": ([P, P, u64, u64, i, i, C.POINTER(ViewOut)], i),
         "vsgm_set_tuning": ([P, C.POINTER(Tuning)], i),
         "vsgm_get_tuning": ([P, C.POINTER(Tuning)], i),
