@@ -188,6 +188,8 @@ def main():
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
     ap.add_argument("--cl-vertices", type=int, default=65_600_000)
     ap.add_argument("--cl-edges", type=float, default=1.81e9, help="target undirected edges of the Chung-Lu graph")
+    ap.add_argument("--cl-sample-s", type=float, default=120.0, help="Chung-Lu: seconds a sampled 4-clique / house pass may grow to")
+    ap.add_argument("--cl-parity-roots", type=int, default=16, help="Chung-Lu: roots whose counts are checked against the oracle")
     ap.add_argument("--workload", choices=["single", "batched", "copy-calibration", "chunglu"], default="single",
                     help="single: the 4-clique/diamond/4-cycle line; batched: many random patterns over k view bins, "
                          "one-at-a-time vs batched+tuned vs learned+batched; copy-calibration: kernels of known HBM "
@@ -449,61 +451,143 @@ def learned_plans(V, args, keys, queries, g):
 
 
 def chunglu_main(args):
-    """Friendster-shaped power-law graph: Chung-Lu with degree exponent 2.3
-    (weights (i+1)^(-1/1.3)), --cl-vertices vertices and about --cl-edges
-    undirected edges after deduplication; --bins view bins by a seeded uniform
-    assignment, each built on the device from the resident graph; the
-    patterns of --patterns (default here: triangle) over every bin. The count
-    is checked equal to the same plans over one whole-graph view."""
+    """Friendster-shaped power-law graph, partitioned and packed as the
+    reference's offline tools do, then matched bin by bin:
+      graph      Chung-Lu with degree exponent 2.3 (weights (i+1)^(-1/1.3)),
+                 --cl-vertices vertices, about --cl-edges undirected edges
+                 after deduplication, seeded;
+      partition  GPU k-means, k = --bins (4), kmeans++ seeding, 10 rounds,
+                 over closed 1-hop neighbourhoods (kmeans.cc);
+      packing    the reference's view packing with -m 10 (bins of at most
+                 2.5e9 view entries) for 2-hop and 3-hop views
+                 (view_packing.cc);
+      views      every bin's view built on the device from the resident
+                 graph, the next bin's build overlapped with the current
+                 bin's match (a second context and stream);
+      patterns   triangle over every source of every 2-hop bin, and again
+                 over the 3-hop bins; 4-clique and house over a seeded
+                 sample of each 2-hop bin's sources that grows 4x while a
+                 pass takes under an eighth of --cl-sample-s (labelled with
+                 its fraction).
+    Counts: the 2-hop and 3-hop triangle totals equal the same plan over one
+    whole-graph view, and all three patterns equal the oracle's on a seeded
+    sample of root vertices."""
     import numpy as np
 
+    import oracle as O
     import vsgm_amd as V
+    from vsgm_amd import partition
 
     assert V.device_count() > 0, "no GPU visible"
-    keys = [k.strip() for k in args.patterns.split(",") if k.strip()] if args.patterns != "kc4,p2,p1" else ["kc3"]
     t = time.perf_counter()
     # duplicate draws between hubs and self loops are removed: about 3-5 % of
     # the draws at this exponent, so 1.05 draws per target edge
     g = V.Graph.chung_lu(args.cl_vertices, int(args.cl_edges * 1.05), 2.3, seed=args.seed)
-    log(f"Chung-Lu: V={g.vertex_count} E={g.edge_count} maxdeg={g.max_degree} generated in {time.perf_counter() - t:.1f}s")
-    plans = [V.Plan.optimize(make_query(V, k)) for k in keys]
-    hop = max(p.hop for p in plans)
-    rd = min(p.root_degree for p in plans)
-    ex = V.PipelineExecutor(0, g.vertex_count, plans)
+    gen_s = time.perf_counter() - t
+    log(f"Chung-Lu: V={g.vertex_count} E={g.edge_count} maxdeg={g.max_degree} generated in {gen_s:.1f}s")
+    keys = ["kc3", "kc4", "p4"]
+    plans = {k: V.Plan.optimize(make_query(V, k)) for k in keys}
+    rd = min(p.root_degree for p in plans.values())
+    hop = max(p.hop for p in plans.values())
+    info = {"generate_s": round(gen_s, 1)}
+
     t = time.perf_counter()
-    ex.set_graph(g)
-    up_s = time.perf_counter() - t
-    assign = np.random.default_rng(args.seed).integers(0, args.bins, size=g.vertex_count).astype(np.int32)
-    t = time.perf_counter()
-    for b in range(args.bins):
-        ex.add_device_view(np.nonzero(assign == b)[0].astype(np.uint32), hop, rd)
-    view_s = time.perf_counter() - t
-    log(f"graph upload {up_s:.1f}s, {args.bins} {hop}-hop device views in {view_s:.2f}s")
-    ex.profile(True)
-    t = time.perf_counter()
-    ex.match_all()
-    match_s = time.perf_counter() - t
-    kstats = ex.kernel_stats()
-    ex.profile(False)
-    counts = ex.total_counts
-    log(f"match over {args.bins} bins: {match_s:.2f}s counts={counts}")
-    whole = V.PipelineExecutor(0, g.vertex_count, plans)
+    assign, km_sizes, obj = partition.kmeans(g, args.bins, iters=10, seed=args.seed)
+    info["kmeans"] = {"k": args.bins, "iters": 10, "sizes": [int(x) for x in km_sizes], "seconds": round(time.perf_counter() - t, 1)}
+    log(f"k-means k={args.bins}: sizes {km_sizes.tolist()} in {info['kmeans']['seconds']}s")
+    packs = {}
+    for h in (2, 3):
+        t = time.perf_counter()
+        try:
+            nb, bin_of, src_n, sizes = V.pack_view_bins(g, h, assign, args.bins, max_bin_entries=int(10e9 / 4))
+            packs[h] = (nb, bin_of)
+            info[f"packing_{h}hop"] = {"bins": nb, "sources": [int(x) for x in src_n], "entries": [int(x) for x in sizes],
+                                       "seconds": round(time.perf_counter() - t, 1)}
+        except V.VsgmError as e:
+            info[f"packing_{h}hop"] = {"error": str(e), "seconds": round(time.perf_counter() - t, 1)}
+        log(f"packing -m 10, {h}-hop: {info[f'packing_{h}hop']}")
+
+    def run_bins(plan_list, bins_sources, view_hop, label):
+        """Every bin's view on the device (the next one prefetched while this one matches)."""
+        ex = V.PipelineExecutor(0, g.vertex_count, plan_list)
+        ex.set_graph(g)
+        ex.profile(True)
+        t0 = time.perf_counter()
+        build = match = 0.0
+        for i, srcs in enumerate(bins_sources):
+            t = time.perf_counter()
+            ex.transfer_device(srcs, view_hop, rd)
+            build += time.perf_counter() - t
+            if i + 1 < len(bins_sources):
+                ex.prefetch_device(bins_sources[i + 1], view_hop, rd)
+            t = time.perf_counter()
+            ex.match()
+            match += time.perf_counter() - t
+        wall = time.perf_counter() - t0
+        kst = ex.kernel_stats()
+        counts = ex.total_counts
+        log(f"{label}: {len(bins_sources)} bins in {wall:.2f}s (waiting for views {build:.2f}s, matching {match:.2f}s) counts {counts}")
+        del ex
+        return counts, wall, match, build, kst
+
+    out = {}
+    whole = V.PipelineExecutor(0, g.vertex_count, [plans["kc3"]])
     whole.transfer_graph(g)
     whole.match()
-    assert whole.total_counts == counts, f"bins {counts} vs whole graph {whole.total_counts}"
-    dom = max(kstats, key=lambda s: s["total_ms"])
+    tri_whole = whole.total_counts[0]
+    del whole
+    log(f"triangles over one whole-graph view: {tri_whole}")
+    kstats_main = None
+    for h in (2, 3):
+        if h not in packs:
+            continue
+        nb, bin_of = packs[h]
+        bins_sources = [np.nonzero(bin_of == b)[0].astype(np.uint32) for b in range(nb)]
+        counts, wall, match, build, kst = run_bins([plans["kc3"]], bins_sources, h, f"triangle, {h}-hop bins")
+        assert counts[0] == tri_whole, f"{h}-hop bins count {counts[0]} triangles, the whole graph {tri_whole}"
+        out[f"triangle_{h}hop"] = {"count": counts[0], "seconds": round(wall, 3), "match_s": round(match, 3),
+                                   "view_wait_s": round(build, 3), "embeddings_per_s": counts[0] / match}
+        if h == 2:
+            kstats_main = kst
+    # 4-clique and house over growing samples of every 2-hop bin's sources
+    rng = np.random.default_rng(args.seed)
+    nb, bin_of = packs[2]
+    order = [rng.permutation(np.nonzero(bin_of == b)[0].astype(np.uint32)) for b in range(nb)]
+    for k in ("kc4", "p4"):
+        frac = 1.0 / 4096
+        best = None
+        while True:
+            sample = [np.sort(o[: max(1, int(len(o) * frac))]) for o in order]
+            counts, wall, match, build, _ = run_bins([plans[k]], sample, plans[k].hop, f"{NAMES[k]}, {frac:.6f} of the sources")
+            best = {"count": counts[0], "sample_fraction": frac, "sources": int(sum(len(x) for x in sample)),
+                    "seconds": round(wall, 3), "match_s": round(match, 3), "embeddings_per_s": counts[0] / match if match else 0.0}
+            if frac >= 1.0 or wall * 4 > args.cl_sample_s / 2 or elapsed() + wall * 4 > args.budget_s:
+                break
+            frac = min(1.0, frac * 4)
+        out[NAMES[k]] = best
+    # parity: all three plans against the oracle on seeded roots
+    roots = np.sort(rng.permutation(g.vertex_count)[: args.cl_parity_roots]).astype(np.uint32)
+    rp, cl = g.csr(copy=False)
+    pe = V.PipelineExecutor(0, g.vertex_count, [plans[k] for k in keys])
+    pe.transfer_graph(g, roots)
+    pe.match()
+    want = [O.count_plan(rp, cl, plans[k].serialize(), sources=roots, threads=min(os.cpu_count() or 1, 16)) for k in keys]
+    parity = {"roots": int(len(roots)), "gpu": pe.total_counts, "oracle": want, "equal": pe.total_counts == want}
+    log(f"parity on {len(roots)} roots: {parity}")
+    assert parity["equal"], f"GPU {pe.total_counts} vs oracle {want} on sampled roots"
+    dom = max(kstats_main, key=lambda s: s["total_ms"])
     achieved = dom["algo_bytes"] / (dom["total_ms"] / 1000.0) / 1e9
+    tri = out["triangle_2hop"]
     line = {
         "metric": "embeddings/sec per pattern, 1 MI355X; learned vs heuristic plan, exact counts",
-        "value": sum(counts) / match_s, "unit": "embeddings/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": match_s * 1000.0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "value": tri["embeddings_per_s"], "unit": "embeddings/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": tri["match_s"] * 1000.0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": f"synthetic: seeded Chung-Lu (seed {args.seed}) generated on the host",
-        "config": {"workload": f"Chung-Lu gamma 2.3, V={g.vertex_count}, E={g.edge_count}, {args.bins} device-built "
-                               f"{hop}-hop view bins, patterns {','.join(keys)}, heuristic plans",
-                   "counts": counts, "view_build_s": view_s, "graph_upload_s": up_s,
-                   "whole_graph_counts_equal": True},
+        "config": {"workload": f"Chung-Lu gamma 2.3, V={g.vertex_count}, E={g.edge_count}, GPU k-means k={args.bins}, "
+                               f"view packing -m 10 (2-hop and 3-hop), device-built views; value: triangles over the 2-hop bins",
+                   "per_pattern": out, "pipeline": info, "whole_graph_triangles": tri_whole, "sampled_root_parity": parity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom["name"],
+                     "frac": round(min(achieved / PEAK_HBM_GBS, 1.0), 4), "traffic": None, "kernel": dom["name"],
                      "launches": dom["launches"], "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 3)},
         "cpu_baseline": None,
     }

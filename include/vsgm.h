@@ -94,6 +94,11 @@ int vsgm_plan_order(const vsgm_plan* p, uint32_t* seq, int cap);
 /* Key/value text of the whole plan (the planner parity fixtures' format);
  * returns the needed size in *needed. */
 int vsgm_plan_serialize(const vsgm_plan* p, char* buf, size_t len, size_t* needed);
+/* Number of leading exec levels two plans share (same operator, vertex,
+ * vertex groups, backward set and conditions; identical partial embeddings
+ * after them), the unit of prefix-shared matching. No counterpart in the
+ * reference. */
+int vsgm_plans_shared_prefix(const vsgm_plan* a, const vsgm_plan* b, int* levels);
 void vsgm_plan_free(vsgm_plan* p);
 
 /* ---- view bins (src/view) ------------------------------------------------ */
@@ -129,10 +134,20 @@ int vsgm_executor_create(int device, uint64_t vertex_count, const vsgm_plan* con
  * "profile" (1 = time every level op), "hub_bytes" (device bytes for the
  * dense membership index of the highest-degree view lists, built at the next
  * transfer; 0 = off, -1 = min(budget/8, 16 GiB), the default),
- * "hub_min_degree" (smallest indexed degree, default 64), and the device
+ * "hub_min_degree" (smallest indexed degree, default 64), the device
  * tuning knobs "isect_kernel", "max_grid", "cold_weight" and "ticket_chunks"
- * (vsgm_tuning in vsgm_hip.h), applied to every plan. */
+ * (vsgm_tuning in vsgm_hip.h), applied to every plan, "prefetch_bytes"
+ * (memory of the background view builder, 0 = a quarter of the budget) and
+ * "share_prefixes" (1 = match the plans of a call as a prefix tree: a level
+ * op that several plans reach through identical levels runs once for all,
+ * sharing the common plan prefixes of queries such as the -km graphlets of
+ * src/query/graphlet.h:15-136; per-plan match times are then not measured,
+ * vsgm_executor_shared_stats reports the whole call's). */
 int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value);
+/* Prefix-shared matching: summed time of the shared calls and the number of
+ * level-op runs saved (no counterpart in the reference, which runs every
+ * plan alone: pipeline_executor.h:83-109). */
+int vsgm_executor_shared_stats(const vsgm_executor* e, double* ms, uint64_t* saved_runs);
 /* Number of view vertices in the current membership index. */
 int vsgm_executor_hub_count(const vsgm_executor* e, uint32_t* n);
 /* Launch shape and batch budget of one plan (0 = the executor option),
@@ -165,6 +180,14 @@ int vsgm_executor_add_device_view(vsgm_executor* e, const uint32_t* sources, uin
  * view from the set_graph graph and replaces every resident view with it
  * (what pm_pipeline -dv 1 does per bin). */
 int vsgm_executor_transfer_device(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree);
+/* Start building a bin's view in the background (a second device context
+ * and a builder thread) so that it overlaps whatever the executor runs next,
+ * typically the current bin's match; the next transfer_device with the same
+ * arguments takes it. The overlap the reference gets from its producer
+ * threads (view materialisation beside Transfer + Match,
+ * src/app/pattern_matching_pipeline.cu:103-172), with the view built on the
+ * device. */
+int vsgm_executor_prefetch_device(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree);
 int vsgm_executor_view_csr(const vsgm_executor* e, int view, uint64_t* row_ptrs, uint32_t* cols, uint64_t cols_cap, uint64_t* entries);
 int vsgm_executor_match_all(vsgm_executor* e, const int* plans, int n_plans);
 int vsgm_executor_capture(vsgm_executor* e, const int* plans, int n_plans, uint64_t* n_nodes);

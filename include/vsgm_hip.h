@@ -52,6 +52,13 @@ int vsgm_device_count(int* count);
 /* Budget accounting: bytes still available under the budget, the budget,
  * and the high-water mark of live bytes (DeviceMemoryInfo::GetAvailableMemorySize). */
 int vsgm_mem_info(const vsgm_ctx* ctx, uint64_t* available, uint64_t* budget, uint64_t* peak);
+/* No counterpart in the reference (one context per consumer thread there).
+ * vsgm_ctx_set_budget changes the budget of a live context (it must cover
+ * the bytes already live); vsgm_ctx_bind_thread makes the context's device
+ * current on the calling thread, for a context driven from a thread other
+ * than the one that created it (the executor's view-builder thread). */
+int vsgm_ctx_set_budget(vsgm_ctx* ctx, uint64_t mem_budget);
+int vsgm_ctx_bind_thread(const vsgm_ctx* ctx);
 /* Pool allocation (CudaContext::Malloc/Free, cuda_context.h:166-210). */
 int vsgm_alloc(vsgm_ctx* ctx, uint64_t bytes, void** out);
 int vsgm_free(vsgm_ctx* ctx, void* ptr);

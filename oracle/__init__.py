@@ -10,6 +10,7 @@ from .checker import (  # noqa: F401
     count_brute,
     triangles,
     four_cliques,
+    four_cliques_ranked,
     diamonds,
     four_cycles,
     count_plan,

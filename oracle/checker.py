@@ -51,7 +51,7 @@ def _load():
         L = C.CDLL(path)
         P = C.c_void_p
         u64 = C.c_uint64
-        for name in ("oracle_triangles", "oracle_four_cliques", "oracle_diamonds", "oracle_four_cycles"):
+        for name in ("oracle_triangles", "oracle_four_cliques", "oracle_four_cliques_ranked", "oracle_diamonds", "oracle_four_cycles"):
             f = getattr(L, name)
             f.argtypes = [u64, P, P, C.c_int]
             f.restype = u64
@@ -99,6 +99,10 @@ def triangles(row_ptrs, cols, threads=0):
 
 def four_cliques(row_ptrs, cols, threads=0):
     return _textbook("oracle_four_cliques", row_ptrs, cols, threads)
+
+
+def four_cliques_ranked(row_ptrs, cols, threads=0):
+    return _textbook("oracle_four_cliques_ranked", row_ptrs, cols, threads)
 
 
 def diamonds(row_ptrs, cols, threads=0):

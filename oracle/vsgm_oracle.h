@@ -16,6 +16,9 @@
  *   oracle_triangles, oracle_four_cliques, oracle_diamonds,
  *   oracle_four_cycles       textbook counters, pinned by the closed forms
  *                            and by oracle_count_brute;
+ *   oracle_four_cliques_ranked  4-cliques over the degree orientation,
+ *                            pinned equal to oracle_four_cliques and to the
+ *                            closed form on K_n;
  *   oracle_count_plan        depth-first restatement of the reference's
  *                            level algorithm driven by a plan
  *                            (src/executor/processor.h:24-196 and
@@ -68,6 +71,7 @@ uint64_t oracle_automorphisms(int n, const uint8_t* adj);
 uint64_t oracle_count_brute(uint64_t V, const uint64_t* row_ptrs, const uint32_t* cols, int n, const uint8_t* adj);
 uint64_t oracle_triangles(uint64_t V, const uint64_t* row_ptrs, const uint32_t* cols, int threads);
 uint64_t oracle_four_cliques(uint64_t V, const uint64_t* row_ptrs, const uint32_t* cols, int threads);
+uint64_t oracle_four_cliques_ranked(uint64_t V, const uint64_t* row_ptrs, const uint32_t* cols, int threads);
 uint64_t oracle_diamonds(uint64_t V, const uint64_t* row_ptrs, const uint32_t* cols, int threads);
 uint64_t oracle_four_cycles(uint64_t V, const uint64_t* row_ptrs, const uint32_t* cols, int threads);
 /* Count of the plan's pattern over roots `sources` (NULL = all vertices).

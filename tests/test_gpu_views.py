@@ -84,3 +84,49 @@ def test_device_view_build_time(vsgm, gpu):
     ex.match_all()
     ex2.match_all()
     assert ex.total_counts == ex2.total_counts
+
+
+def test_prefetched_views_count_like_built_views(vsgm, oracle, gpu):
+    """Each bin's view built in the background (second context and stream)
+    while the previous bin matches: per-bin counts equal the synchronous
+    builds', the total equals the oracle's, a prefetch of another bin is
+    dropped (the transfer builds its own view), and the executor can be
+    destroyed with a prefetch in flight."""
+    g = vsgm.Graph.rmat(12, 16, seed=11)
+    V = g.vertex_count
+    plans = [vsgm.Plan.optimize(q) for q in (vsgm.Query.clique(4), vsgm.Query.preset(1), vsgm.Query.preset(2))]
+    hop = max(p.hop for p in plans)
+    rd = min(p.root_degree for p in plans)
+    rp, cl = g.csr()
+    want = [oracle.count_plan(rp, cl, p.serialize()) for p in plans]
+    assign = np.random.default_rng(5).integers(0, 6, size=V).astype(np.int32)
+    bins = [np.nonzero(assign == b)[0].astype(np.uint32) for b in range(6)]
+    sync = vsgm.PipelineExecutor(0, V, plans)
+    sync.set_graph(g)
+    per_bin = []
+    for s in bins:
+        sync.reset()
+        sync.transfer_device(s, hop, rd)
+        sync.match()
+        per_bin.append(sync.total_counts)
+    assert [sum(c[i] for c in per_bin) for i in range(len(plans))] == want
+    ex = vsgm.PipelineExecutor(0, V, plans)
+    ex.set_graph(g)
+    print("[gpu] prefetched device views, 6 bins of R-MAT s12", flush=True)
+    ex.transfer_device(bins[0], hop, rd)
+    for b in range(6):
+        if b + 1 < 6:
+            ex.prefetch_device(bins[b + 1], hop, rd)
+        ex.reset()
+        ex.match()
+        assert ex.total_counts == per_bin[b], b
+        if b + 1 < 6:
+            ex.transfer_device(bins[b + 1], hop, rd)
+    # a prefetch of bin 2 followed by a transfer of bin 3: bin 3's own view
+    ex.prefetch_device(bins[2], hop, rd)
+    ex.transfer_device(bins[3], hop, rd)
+    ex.reset()
+    ex.match()
+    assert ex.total_counts == per_bin[3]
+    ex.prefetch_device(bins[4], hop, rd)
+    del ex

@@ -111,14 +111,23 @@ def test_textbook_counters_against_brute_force(oracle, seed):
     rp, cl = random_graph(24, 0.35, seed)
     assert oracle.triangles(rp, cl) == oracle.count_brute(rp, cl, TRI)
     assert oracle.four_cliques(rp, cl) == oracle.count_brute(rp, cl, K4)
+    assert oracle.four_cliques_ranked(rp, cl) == oracle.count_brute(rp, cl, K4)
     assert oracle.diamonds(rp, cl) == oracle.count_brute(rp, cl, DIAMOND)
     assert oracle.four_cycles(rp, cl) == oracle.count_brute(rp, cl, SQUARE)
+
+
+@pytest.mark.parametrize("scale", [8, 11, 13])
+def test_ranked_four_cliques_equal_the_id_ordered_counter(vsgm, oracle, scale):
+    # skewed R-MAT graphs, where degree order and id order differ most
+    rp, cl = vsgm.Graph.rmat(scale, 16, seed=scale).csr()
+    assert oracle.four_cliques_ranked(rp, cl) == oracle.four_cliques(rp, cl)
 
 
 def test_textbook_counters_closed_forms(oracle):
     rp, cl = complete(9)
     assert oracle.triangles(rp, cl) == math.comb(9, 3)
     assert oracle.four_cliques(rp, cl) == math.comb(9, 4)
+    assert oracle.four_cliques_ranked(rp, cl) == math.comb(9, 4)
     assert oracle.diamonds(rp, cl) == 6 * math.comb(9, 4)
     assert oracle.four_cycles(rp, cl) == 3 * math.comb(9, 4)
 

@@ -175,3 +175,69 @@ def test_from_order_hop_covers_every_list_it_reads(vsgm, oracle):
     n, _, a, _ = q.info()
     assert max(vsgm.Plan.from_order(q, o).hop for o in itertools.permutations(range(4))
                if all(any(a[o[i], o[j]] for j in range(i)) for i in range(1, 4))) == 3
+
+
+def _plan_fields(text):
+    """The serialized plan's fields as lists of bracket groups (strings)."""
+    out = {}
+    for line in text.splitlines():
+        if ":" not in line:
+            continue
+        key, val = line.split(":", 1)
+        if key == "exec":
+            out[key] = re.findall(r"\((\w+),(\d+)\)", val)
+        elif val.startswith("["):
+            out[key] = re.findall(r"\[([^\]]*)\]", val)
+    return out
+
+
+def _signature(f, level):
+    """Level signature restated from the plan text: operator, vertex,
+    materialised and pending groups, and what the op applies (backward set
+    and compute conditions, with the condition against the one pending set
+    of a COMPUTE_COUNT; materialise or filter conditions; the condition
+    between the two sets of a COUNT)."""
+    op, u = f["exec"][level]
+    u = int(u)
+    cu = [int(x) for x in f["computed_unmaterialized"][level].split(",") if x]
+
+    def cond(a, b):  # first entry of a's order list naming b, else NON_EQUAL
+        for c in re.findall(r"(\w+)\((\d+)\)", f["order"][a]):
+            if int(c[1]) == b:
+                return c[0]
+        return "NON_EQUAL"
+
+    sig = [op, u, f["materialized"][level], f["computed_unmaterialized"][level]]
+    if op in ("COMPUTE", "COMPUTE_COUNT", "COMPUTE_PATH_COUNT"):
+        sig += [f["backward"][u], f["compute_cond"][u]]
+        if op == "COMPUTE_COUNT" and len(cu) == 1:
+            sig.append(cond(u, cu[0]))
+    elif op == "MATERIALIZE":
+        sig.append(f["materialize_cond"][u])
+    elif op == "FILTER_COMPUTE":
+        sig.append(f["filter_cond"][u])
+    elif op == "COUNT" and len(cu) == 2:
+        sig.append(cond(cu[0], cu[1]))
+    return sig
+
+
+def test_shared_prefix_matches_the_plan_text(vsgm):
+    """Prefix-shared matching groups plans by their level signatures: the
+    library's shared-prefix length equals the one restated from the plan
+    texts for every pair of graphlet (3..5), preset and clique plans, and a
+    plan shares all its levels with itself."""
+    qs = [q for k in (3, 4, 5) for q in vsgm.Query.graphlets(k)]
+    qs += [vsgm.Query.preset(k) for k in range(19)] + [vsgm.Query.clique(k) for k in range(3, 8)]
+    plans = [vsgm.Plan.optimize(q) for q in qs]
+    fields = [_plan_fields(p.serialize()) for p in plans]
+    sigs = [[_signature(f, l) for l in range(len(f["exec"]))] for f in fields]
+    shared_somewhere = 0
+    for i, a in enumerate(plans):
+        assert a.shared_prefix(a) == len(sigs[i])
+        for j, b in enumerate(plans):
+            n = 0
+            while n < min(len(sigs[i]), len(sigs[j])) and sigs[i][n] == sigs[j][n]:
+                n += 1
+            assert a.shared_prefix(b) == n, (i, j)
+            shared_somewhere += i != j and n >= 2
+    assert shared_somewhere > 0  # some plans do share more than the root level

@@ -103,6 +103,7 @@ def _declare(L):
         "vsgm_plan_from_order": ([P, P, C.c_int, PP], C.c_int),
         "vsgm_plan_info": ([P, C.POINTER(C.c_int), C.POINTER(u64), C.POINTER(C.c_int)], C.c_int),
         "vsgm_plan_op": ([P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "vsgm_plans_shared_prefix": ([P, P, C.POINTER(C.c_int)], C.c_int),
         "vsgm_plan_order": ([P, P, C.c_int], C.c_int),
         "vsgm_plan_serialize": ([P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
         "vsgm_plan_free": ([P], None),
@@ -125,6 +126,7 @@ def _declare(L):
         "vsgm_executor_match": ([P, C.c_int], C.c_int),
         "vsgm_executor_counts": ([P, P, C.c_int], C.c_int),
         "vsgm_executor_match_ms": ([P, P, C.c_int], C.c_int),
+        "vsgm_executor_shared_stats": ([P, C.POINTER(C.c_double), C.POINTER(u64)], C.c_int),
         "vsgm_executor_reset": ([P], C.c_int),
         "vsgm_executor_level_stats": ([P, C.c_int, P, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_peak_bytes": ([P, C.POINTER(u64)], C.c_int),
@@ -136,6 +138,7 @@ def _declare(L):
         "vsgm_executor_set_graph": ([P, P], C.c_int),
         "vsgm_executor_add_device_view": ([P, P, u64, C.c_int, u64, C.POINTER(C.c_int)], C.c_int),
         "vsgm_executor_transfer_device": ([P, P, u64, C.c_int, u64], C.c_int),
+        "vsgm_executor_prefetch_device": ([P, P, u64, C.c_int, u64], C.c_int),
         "vsgm_executor_view_csr": ([P, C.c_int, P, P, u64, C.POINTER(u64)], C.c_int),
         "vsgm_executor_match_all": ([P, P, C.c_int], C.c_int),
         "vsgm_executor_capture": ([P, P, C.c_int, C.POINTER(u64)], C.c_int),
@@ -379,6 +382,12 @@ class Plan(_Handle):
     def root_degree(self) -> int:
         return self._info()[1]
 
+    def shared_prefix(self, other: "Plan") -> int:
+        """Leading exec levels this plan shares with `other` (the unit of prefix-shared matching)."""
+        n = C.c_int()
+        _check(lib().vsgm_plans_shared_prefix(self._h, other._h, C.byref(n)), "shared prefix")
+        return n.value
+
     @property
     def ops(self):
         out = []
@@ -525,6 +534,12 @@ class PipelineExecutor(_Handle):
         _check(lib().vsgm_executor_match_ms(self._h, _ptr(out), len(self.plans)), "match_ms")
         return [float(x) for x in out]
 
+    def shared_stats(self) -> dict:
+        """Prefix-shared matching: summed time of the shared calls and the level-op runs saved."""
+        ms, runs = C.c_double(), u64()
+        _check(lib().vsgm_executor_shared_stats(self._h, C.byref(ms), C.byref(runs)), "shared_stats")
+        return {"ms": ms.value, "saved_runs": runs.value}
+
     def level_stats(self, plan_index: int) -> List[dict]:
         arr = (LevelStat * 64)()
         n = C.c_int()
@@ -569,6 +584,12 @@ class PipelineExecutor(_Handle):
         """Build one bin's view on the device and make it the only resident view (like transfer)."""
         s = np.ascontiguousarray(np.asarray(sources, dtype=np.uint32))
         _check(lib().vsgm_executor_transfer_device(self._h, _ptr(s), len(s), hop, root_degree), "transfer_device")
+
+    def prefetch_device(self, sources, hop: int, root_degree: int):
+        """Start building a bin's view in the background (second device context and stream); the next
+        transfer_device with the same arguments takes it, so the build overlaps the current match."""
+        s = np.ascontiguousarray(np.asarray(sources, dtype=np.uint32))
+        _check(lib().vsgm_executor_prefetch_device(self._h, _ptr(s), len(s), hop, root_degree), "prefetch_device")
 
     def view_csr(self, view: int, vertex_count: int):
         n = u64()

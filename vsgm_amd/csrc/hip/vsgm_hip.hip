@@ -3150,6 +3150,19 @@ int vsgm_mem_info(const vsgm_ctx* ctx, uint64_t* available, uint64_t* budget, ui
   return VSGM_OK;
 }
 
+int vsgm_ctx_set_budget(vsgm_ctx* ctx, uint64_t mem_budget) {
+  if (!ctx) return VSGM_ERR_ARG;
+  if (mem_budget < ctx->live)
+    return fail(ctx, VSGM_ERR_ARG, "budget %llu is below the %llu bytes live", (unsigned long long)mem_budget, (unsigned long long)ctx->live);
+  ctx->budget = mem_budget;
+  return VSGM_OK;
+}
+
+int vsgm_ctx_bind_thread(const vsgm_ctx* ctx) {
+  if (!ctx) return VSGM_ERR_ARG;
+  return hipSetDevice(ctx->device) == hipSuccess ? VSGM_OK : VSGM_ERR_HIP;
+}
+
 int vsgm_alloc(vsgm_ctx* ctx, uint64_t bytes, void** out) {
   if (!ctx || !out) return VSGM_ERR_ARG;
   return pool_alloc(ctx, bytes, out);

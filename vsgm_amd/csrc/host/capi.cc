@@ -240,6 +240,13 @@ int vsgm_plan_info(const vsgm_plan* p, int* hop, uint64_t* root_degree, int* n_o
   });
 }
 
+int vsgm_plans_shared_prefix(const vsgm_plan* a, const vsgm_plan* b, int* levels) {
+  return Guard([&] {
+    VSGM_CHECK(a && b && levels, "null argument");
+    *levels = (int)vsgm::SharedPrefix(*a->p, *b->p);
+  });
+}
+
 int vsgm_plan_op(const vsgm_plan* p, int i, int* op, int* vertex) {
   return Guard([&] {
     VSGM_CHECK(p, "null argument");
@@ -453,6 +460,8 @@ int vsgm_executor_set_option(vsgm_executor* e, const char* key, double value) {
     else if (k == "profile") o.profile = value != 0;
     else if (k == "hub_bytes") o.hub_bytes = value < 0 ? ~0ull : (uint64_t)value;
     else if (k == "hub_min_degree") o.hub_min_degree = (uint64_t)value;
+    else if (k == "prefetch_bytes") o.prefetch_bytes = (uint64_t)value;
+    else if (k == "share_prefixes") o.share_prefixes = value != 0;
     else if (k == "isect_kernel" || k == "max_grid" || k == "cold_weight" || k == "ticket_chunks") {
       // applies to every plan (the per-plan tuning of the autotuner refines it)
       vsgm_tuning t;
@@ -520,6 +529,13 @@ int vsgm_executor_transfer_device(vsgm_executor* e, const uint32_t* sources, uin
     VSGM_CHECK(e, "null argument");
     VSGM_CHECK(e && (sources || n_sources == 0), "bad arguments");
     e->e->TransferDevice(std::vector<vsgm::uintV>(sources, sources + n_sources), hop, root_degree);
+  });
+}
+
+int vsgm_executor_prefetch_device(vsgm_executor* e, const uint32_t* sources, uint64_t n_sources, int hop, uint64_t root_degree) {
+  return Guard([&] {
+    VSGM_CHECK(e && (sources || n_sources == 0), "null argument");
+    e->e->PrefetchDeviceView(std::vector<vsgm::uintV>(sources, sources + n_sources), hop, root_degree);
   });
 }
 
@@ -617,6 +633,14 @@ int vsgm_executor_match_ms(const vsgm_executor* e, double* ms, int n) {
     const auto& c = e->e->GetMatchMs();
     VSGM_CHECK(n >= (int)c.size(), "buffer too small");
     for (size_t i = 0; i < c.size(); ++i) ms[i] = c[i];
+  });
+}
+
+int vsgm_executor_shared_stats(const vsgm_executor* e, double* ms, uint64_t* saved_runs) {
+  return Guard([&] {
+    VSGM_CHECK(e && ms && saved_runs, "null argument");
+    *ms = e->e->SharedMatchMs();
+    *saved_runs = e->e->SharedLevelRuns();
   });
 }
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <sstream>
 
 namespace vsgm {
 
@@ -100,11 +101,18 @@ PipelineExecutor::PipelineExecutor(int device, uint64_t vertex_count, std::vecto
 
 PipelineExecutor::~PipelineExecutor() {
   vsgm_graph_release(ctx_);
+  try {
+    JoinPrefetch();
+  } catch (const std::exception&) {
+    // a failed background build leaves nothing behind
+  }
+  prefetched_.reset();
   ClearViews();
   graph_rp_.reset();
   graph_cols_.reset();
   if (d_counts_) vsgm_free(ctx_, d_counts_);
   if (h_counts_) vsgm_host_free(h_counts_);
+  vsgm_ctx_destroy(builder_);  // after every view it built is gone
   vsgm_ctx_destroy(ctx_);
 }
 
@@ -155,6 +163,9 @@ void PipelineExecutor::Check(int status, const char* what) const {
 }
 
 void PipelineExecutor::ClearViews() {
+  // views built by the builder context are freed into its pool, which a
+  // background build must not be using meanwhile
+  JoinPrefetch();
   ReleaseGraph();
   for (auto& v : views_) vsgm_hub_index_free(ctx_, &v.hub);
   views_.clear();
@@ -215,6 +226,8 @@ void PipelineExecutor::IndexView(View& v, const uint64_t* row_ptrs, uint64_t hub
 
 void PipelineExecutor::SetGraph(const Graph& g) {
   VSGM_CHECK(g.vertex_count == vertex_count_, "graph vertex count does not match the executor");
+  JoinPrefetch();  // a background build reads the resident graph
+  prefetched_.reset();
   ReleaseGraph();
   graph_rp_ = std::make_shared<DeviceBuffer>(ctx_, (vertex_count_ + 1) * sizeof(uintE));
   graph_cols_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(g.cols.size(), 1) * sizeof(uintV));
@@ -232,29 +245,86 @@ size_t PipelineExecutor::AddDeviceView(const std::vector<uintV>& sources, int ho
 }
 
 void PipelineExecutor::TransferDevice(const std::vector<uintV>& sources, int hop, uint64_t root_degree) {
+  JoinPrefetch();
+  std::unique_ptr<BuiltView> b = std::move(prefetched_);
+  const bool hit = b && hop == prefetch_hop_ && root_degree == prefetch_root_degree_ && sources == prefetch_sources_;
+  if (!hit) b.reset();  // a prefetch of another bin is dropped
   ClearViews();
-  views_.push_back(BuildDeviceView(sources, hop, root_degree, HubLimit(ctx_, opt_.hub_bytes, 1)));
+  if (hit) {
+    IndexView(b->view, b->h_row_ptrs.data(), HubLimit(ctx_, opt_.hub_bytes, 1));
+    views_.push_back(std::move(b->view));
+  } else {
+    views_.push_back(BuildDeviceView(sources, hop, root_degree, HubLimit(ctx_, opt_.hub_bytes, 1)));
+  }
   cur_ = 0;
+}
+
+void PipelineExecutor::PrefetchDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree) {
+  VSGM_CHECK(graph_rp_ && graph_cols_, "device view before SetGraph");
+  CheckSources(sources.data(), sources.size());
+  JoinPrefetch();
+  prefetched_.reset();
+  if (!builder_) {
+    uint64_t budget = 0, available = 0;
+    vsgm_mem_info(ctx_, &available, &budget, nullptr);
+    const uint64_t take = opt_.prefetch_bytes ? opt_.prefetch_bytes : budget / 4;
+    VSGM_CHECK(take < available, "the view builder's memory does not fit in what the executor has left");
+    Check(vsgm_ctx_set_budget(ctx_, budget - take), "budget split for the view builder");
+    vsgm_ctx* b = nullptr;
+    if (vsgm_ctx_create(device_, take, &b) != VSGM_OK) {
+      vsgm_ctx_set_budget(ctx_, budget);
+      throw Error("cannot create the view builder's device context");
+    }
+    builder_ = b;
+  }
+  prefetch_sources_ = sources;
+  prefetch_hop_ = hop;
+  prefetch_root_degree_ = root_degree;
+  prefetch_ = std::async(std::launch::async, [this, sources, hop, root_degree] {
+    if (vsgm_ctx_bind_thread(builder_) != VSGM_OK) throw Error("view builder thread: cannot select the device");
+    return BuildViewOn(builder_, sources, hop, root_degree);
+  });
+  prefetch_pending_ = true;
+}
+
+void PipelineExecutor::JoinPrefetch() {
+  if (!prefetch_pending_) return;
+  prefetch_pending_ = false;
+  prefetched_.reset(new BuiltView(prefetch_.get()));  // rethrows a failed build
+}
+
+// The view of one bin on context ctx (its stream and pool): BFS levels over
+// the resident graph, the copy of the selected lists, and the row_ptrs read
+// back for the membership index; returns once the device work is complete.
+PipelineExecutor::BuiltView PipelineExecutor::BuildViewOn(vsgm_ctx* ctx, const std::vector<uintV>& sources, int hop,
+                                                          uint64_t root_degree) const {
+  auto check = [ctx](int status, const char* what) {
+    if (status != VSGM_OK) throw Error(std::string(what) + " failed: " + vsgm_last_error(ctx));
+  };
+  BuiltView b;
+  View& v = b.view;
+  v.sources = std::make_shared<DeviceBuffer>(ctx, std::max<uint64_t>(sources.size(), 1) * sizeof(uintV));
+  v.sources_num = sources.size();
+  check(vsgm_h2d(ctx, v.sources->get(), sources.data(), sources.size() * sizeof(uintV)), "sources upload");
+  vsgm_view_out o;
+  check(vsgm_view_build(ctx, (const uint64_t*)graph_rp_->get(), (const uint32_t*)graph_cols_->get(), vertex_count_,
+                        (const uint32_t*)v.sources->get(), sources.size(), hop, root_degree, &o),
+        "view build");
+  v.row_ptrs = std::make_shared<DeviceBuffer>(ctx, (void*)o.row_ptrs);
+  v.cols = std::make_shared<DeviceBuffer>(ctx, (void*)o.cols);
+  b.h_row_ptrs.resize(vertex_count_ + 1);
+  check(vsgm_d2h(ctx, b.h_row_ptrs.data(), o.row_ptrs, b.h_row_ptrs.size() * sizeof(uint64_t)), "view row_ptrs readback");
+  check(vsgm_ctx_sync(ctx), "view build");
+  return b;
 }
 
 PipelineExecutor::View PipelineExecutor::BuildDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree,
                                                          uint64_t hub_limit) {
   VSGM_CHECK(graph_rp_ && graph_cols_, "device view before SetGraph");
   CheckSources(sources.data(), sources.size());
-  View v;
-  v.sources = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(sources.size(), 1) * sizeof(uintV));
-  v.sources_num = sources.size();
-  Check(vsgm_h2d(ctx_, v.sources->get(), sources.data(), sources.size() * sizeof(uintV)), "sources upload");
-  vsgm_view_out o;
-  Check(vsgm_view_build(ctx_, (const uint64_t*)graph_rp_->get(), (const uint32_t*)graph_cols_->get(), vertex_count_,
-                        (const uint32_t*)v.sources->get(), sources.size(), hop, root_degree, &o),
-        "view build");
-  v.row_ptrs = std::make_shared<DeviceBuffer>(ctx_, (void*)o.row_ptrs);
-  v.cols = std::make_shared<DeviceBuffer>(ctx_, (void*)o.cols);
-  std::vector<uint64_t> rp(vertex_count_ + 1);
-  Check(vsgm_d2h(ctx_, rp.data(), o.row_ptrs, rp.size() * sizeof(uint64_t)), "view row_ptrs readback");
-  IndexView(v, rp.data(), hub_limit);
-  return v;
+  BuiltView b = BuildViewOn(ctx_, sources, hop, root_degree);
+  IndexView(b.view, b.h_row_ptrs.data(), hub_limit);
+  return std::move(b.view);
 }
 
 void PipelineExecutor::ViewCsr(size_t view, std::vector<uint64_t>& row_ptrs, std::vector<uint32_t>& cols) const {
@@ -315,6 +385,52 @@ void PipelineExecutor::MatchAll(const std::vector<int>& which) {
 // mode times each plan with HIP events and reads the counts back once;
 // capture mode (tape_mode_ 2) leaves a copy of the counts into the pinned
 // slots as the graph's last node.
+std::string LevelSignature(const Plan& p, size_t level) {
+  const auto& e = p.GetExecuteOperations().at(level);
+  std::ostringstream s;
+  auto group = [&s](const std::vector<uintV>& g) {
+    s << "[";
+    for (auto v : g) s << v << ",";
+    s << "]";
+  };
+  auto conds = [&s](const std::vector<CondType>& c) {
+    s << "{";
+    for (const auto& x : c) s << (int)x.first << ":" << x.second << ",";
+    s << "}";
+  };
+  const auto& mat = p.GetMaterializedVertices()[level];
+  const auto& cu = p.GetComputedUnmaterializedVertices()[level];
+  s << (int)e.first << " " << e.second << " ";
+  group(mat);
+  group(cu);
+  switch (e.first) {
+    case COMPUTE:
+    case COMPUTE_COUNT:
+    case COMPUTE_PATH_COUNT:
+      group(p.GetBackwardConnectivity()[e.second]);
+      conds(p.GetComputeConditions()[e.second]);
+      if (e.first == COMPUTE_COUNT && cu.size() == 1) s << " o" << (int)p.ConditionBetween(e.second, cu[0]);
+      break;
+    case MATERIALIZE:
+      conds(p.GetMaterializeConditions()[e.second]);
+      break;
+    case FILTER_COMPUTE:
+      conds(p.GetFilterConditions()[e.second]);
+      break;
+    case COUNT:
+      if (cu.size() == 2) s << " o" << (int)p.ConditionBetween(cu[0], cu[1]);
+      break;
+  }
+  return s.str();
+}
+
+size_t SharedPrefix(const Plan& a, const Plan& b) {
+  const size_t n = std::min(a.GetExecuteOperations().size(), b.GetExecuteOperations().size());
+  size_t l = 0;
+  while (l < n && LevelSignature(a, l) == LevelSignature(b, l)) ++l;
+  return l;
+}
+
 void PipelineExecutor::MatchPlans(const std::vector<size_t>& views, const std::vector<int>& which) {
   std::vector<int> sel = which;
   if (sel.empty())
@@ -322,20 +438,53 @@ void PipelineExecutor::MatchPlans(const std::vector<size_t>& views, const std::v
   for (int i : sel) VSGM_CHECK(i >= 0 && (size_t)i < plans_.size(), "plan index out of range");
   const bool capturing = tape_mode_ == 2;
   Check(vsgm_fill_u64(ctx_, d_counts_, plans_.size(), 0), "count reset");
-  for (int i : sel) {
-    plan_ = plans_[i];
-    plan_index_ = i;
-    d_count_ = d_counts_ + i;
-    ApplyTuning(i);
-    if (!capturing) Check(vsgm_timer_start(ctx_, 0), "timer");
-    for (size_t v : views) {
-      view_ = &views_[v];
-      ProcessLevel(0, Level{});
+  std::vector<std::pair<int, int>> copies;  // (plan, identical earlier plan): counted once, copied
+  if (opt_.share_prefixes && sel.size() > 1) {
+    std::vector<int> distinct;
+    for (int i : sel) {
+      int same = -1;
+      const size_t n = plans_[i]->GetExecuteOperations().size();
+      for (int j : distinct)
+        if (plans_[j]->GetExecuteOperations().size() == n && SharedPrefix(*plans_[i], *plans_[j]) == n) same = j;
+      if (same < 0) distinct.push_back(i);
+      else copies.emplace_back(i, same);
     }
+    if (!capturing) Check(vsgm_timer_start(ctx_, 0), "timer");
+    sharing_ = true;
+    try {
+      for (size_t v : views) {
+        view_ = &views_[v];
+        group_ = distinct;
+        ProcessShared(0, Level{});
+      }
+    } catch (...) {
+      sharing_ = false;
+      group_.clear();
+      throw;
+    }
+    sharing_ = false;
+    group_.clear();
     if (!capturing) {
       float ms = 0;
       Check(vsgm_timer_stop(ctx_, 0, &ms), "timer");
-      match_ms_[i] += ms;
+      shared_ms_ += ms;
+    }
+  } else {
+    for (int i : sel) {
+      plan_ = plans_[i];
+      plan_index_ = i;
+      d_count_ = d_counts_ + i;
+      ApplyTuning(i);
+      if (!capturing) Check(vsgm_timer_start(ctx_, 0), "timer");
+      for (size_t v : views) {
+        view_ = &views_[v];
+        ProcessLevel(0, Level{});
+      }
+      if (!capturing) {
+        float ms = 0;
+        Check(vsgm_timer_stop(ctx_, 0, &ms), "timer");
+        match_ms_[i] += ms;
+      }
     }
   }
   plan_ = nullptr;
@@ -346,7 +495,50 @@ void PipelineExecutor::MatchPlans(const std::vector<size_t>& views, const std::v
   }
   std::vector<uint64_t> c(plans_.size(), 0);
   Check(vsgm_d2h(ctx_, c.data(), d_counts_, c.size() * sizeof(uint64_t)), "count readback");
+  for (const auto& [i, j] : copies) c[i] = c[j];
   for (int i : sel) total_counts_[i] += c[i];
+}
+
+// Level `level` for every plan of group_ (all identical before it): plans
+// whose op at this level is the same run it once, as a subgroup led by its
+// first plan, and descend together.
+void PipelineExecutor::ProcessShared(size_t level, const Level& parent) {
+  const std::vector<int> group = group_;
+  const Plan* saved_plan = plan_;
+  const size_t saved_index = plan_index_;
+  uint64_t* saved_count = d_count_;
+  std::vector<std::string> keys;
+  std::vector<std::vector<int>> subs;
+  for (int p : group) {
+    const std::string k = LevelSignature(*plans_[p], level);
+    size_t i = 0;
+    while (i < keys.size() && keys[i] != k) ++i;
+    if (i == keys.size()) {
+      keys.push_back(k);
+      subs.emplace_back();
+    }
+    subs[i].push_back(p);
+  }
+  for (const auto& sub : subs) {
+    group_ = sub;
+    plan_ = plans_[sub[0]];
+    plan_index_ = sub[0];
+    d_count_ = d_counts_ + sub[0];
+    ApplyTuning(sub[0]);
+    if (sub.size() > 1) shared_runs_ += sub.size() - 1;
+    ProcessLevel(level, parent);
+  }
+  // back to the caller's plan (its next batch runs its own level op)
+  group_ = group;
+  plan_ = saved_plan;
+  plan_index_ = saved_index;
+  d_count_ = saved_count;
+  if (plan_) ApplyTuning(plan_index_);
+}
+
+void PipelineExecutor::Descend(size_t level, const Level& parent) {
+  if (sharing_) ProcessShared(level, parent);
+  else ProcessLevel(level, parent);
 }
 
 uint64_t PipelineExecutor::CaptureAll(const std::vector<int>& which) {
@@ -503,7 +695,7 @@ void PipelineExecutor::ProcessLevel(size_t level, const Level& parent) {
       stat.batches++;
       stat.paths += child.path_num;
       stat.outputs += child.path_num;
-      ProcessLevel(1, child);
+      Descend(1, child);
     }
     return;
   }
@@ -646,7 +838,7 @@ void PipelineExecutor::RunBatch(size_t level, const Level& slice) {
     Check(vsgm_timer_stop(ctx_, 1, &ms), "timer");
     stat.ms += ms;
   }
-  if (next) ProcessLevel(level + 1, child);
+  if (next) Descend(level + 1, child);
 }
 
 }  // namespace vsgm

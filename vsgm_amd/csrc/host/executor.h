@@ -10,6 +10,7 @@
 // at the end of Match.
 #pragma once
 
+#include <future>
 #include <memory>
 #include <string>
 #include <vector>
@@ -41,7 +42,25 @@ struct ExecutorOptions {
   // vertices of view degree >= hub_min_degree.
   uint64_t hub_bytes = ~0ull;
   uint64_t hub_min_degree = 64;
+  // Memory of the view-builder context used by PrefetchDeviceView (0: a
+  // quarter of the executor's budget), taken out of the executor's budget
+  // when the first prefetch starts.
+  uint64_t prefetch_bytes = 0;
+  // Match the plans of one call as a prefix tree: a level op that several
+  // plans reach through identical levels runs once per batch for all of
+  // them (plan-prefix sharing across queries, e.g. the -km graphlets).
+  // Per-plan match times are then not measured (SharedMatchMs holds the
+  // whole call's), and shared levels' statistics go to the first plan.
+  bool share_prefixes = false;
 };
+
+// What determines a plan's level op and the state it leaves: the operator,
+// its vertex, the materialised and pending vertex groups, and the backward
+// set and conditions the op applies. Plans whose signatures agree on levels
+// 0..l hold identical partial embeddings after level l.
+std::string LevelSignature(const Plan& p, size_t level);
+// Number of leading levels two plans share.
+size_t SharedPrefix(const Plan& a, const Plan& b);
 
 class DeviceBuffer;
 
@@ -78,8 +97,20 @@ class PipelineExecutor {
   void SetGraph(const Graph& g);
   size_t AddDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree);
   // Build one bin's view on the device like AddDeviceView, replacing every
-  // resident view (the device-side counterpart of Transfer).
+  // resident view (the device-side counterpart of Transfer). A view
+  // prefetched with the same arguments is taken instead of building it.
   void TransferDevice(const std::vector<uintV>& sources, int hop, uint64_t root_degree);
+  // Start building a bin's view in the background: a builder thread drives
+  // a second device context (its own stream, same priority, and its own
+  // memory pool) while this thread goes on (typically with the current
+  // bin's Match), so the next bin's view build overlaps the matching. The
+  // next TransferDevice with the same arguments waits for it and installs
+  // it (its membership index is built then, on the executor's context).
+  // Replaces the reference's producer/consumer overlap of host view
+  // materialisation with Transfer + Match (pattern_matching_pipeline.cu:
+  // 103-172), here with both the build and the match on the device.
+  void PrefetchDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree);
+  bool PrefetchPending() const { return prefetch_pending_; }
   // Host copy of a resident view's CSR.
   void ViewCsr(size_t view, std::vector<uint64_t>& row_ptrs, std::vector<uint32_t>& cols) const;
   // Enumerate every plan over the current view (pipeline_executor.h:83-109).
@@ -101,6 +132,8 @@ class PipelineExecutor {
   const std::vector<uintC>& GetTotalCounts() const { return total_counts_; }
   const std::vector<std::vector<LevelStat>>& GetLevelStats() const { return level_stats_; }
   const std::vector<double>& GetMatchMs() const { return match_ms_; }  // per plan, summed over Match calls
+  double SharedMatchMs() const { return shared_ms_; }                   // summed over prefix-shared Match calls
+  uint64_t SharedLevelRuns() const { return shared_runs_; }             // level ops run once for several plans
   void ResetCounts();
   void SetPlans(std::vector<const Plan*> plans);
   vsgm_ctx* Context() { return ctx_; }
@@ -116,6 +149,13 @@ class PipelineExecutor {
     vsgm_hub_index hub{nullptr, nullptr, 0, 0};
     bool loop_free = false;  // no v in adj(v): '!=' against backward vertices is implied
   };
+  // A device view built on some context, before its membership index.
+  struct BuiltView {
+    View view;
+    std::vector<uint64_t> h_row_ptrs;
+  };
+  BuiltView BuildViewOn(vsgm_ctx* ctx, const std::vector<uintV>& sources, int hop, uint64_t root_degree) const;
+  void JoinPrefetch();
   void ClearViews();
   void IndexView(View& v, const uint64_t* h_row_ptrs, uint64_t hub_limit);
   View BuildDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree, uint64_t hub_limit);
@@ -123,6 +163,8 @@ class PipelineExecutor {
                   uint64_t hub_limit);
   void MatchPlans(const std::vector<size_t>& views, const std::vector<int>& which);
   void ProcessLevel(size_t level, const Level& parent);
+  void Descend(size_t level, const Level& parent);     // next level: of plan_, or of every plan in group_
+  void ProcessShared(size_t level, const Level& parent);
   void RunBatch(size_t level, const Level& slice);
   uint64_t LevelBatchBytes(size_t level);
   void FillArgs(size_t level, const Level& l, vsgm_level_args& a) const;
@@ -135,6 +177,16 @@ class PipelineExecutor {
   std::vector<const Plan*> plans_;
   ExecutorOptions opt_;
   vsgm_ctx* ctx_ = nullptr;
+
+  // view prefetch: the builder context (created on the first prefetch), the
+  // build in flight and its arguments; a joined build waits in prefetched_
+  vsgm_ctx* builder_ = nullptr;
+  std::future<BuiltView> prefetch_;
+  bool prefetch_pending_ = false;
+  std::unique_ptr<BuiltView> prefetched_;
+  std::vector<uintV> prefetch_sources_;
+  int prefetch_hop_ = 0;
+  uint64_t prefetch_root_degree_ = 0;
 
   std::vector<View> views_;
   size_t cur_ = 0;
@@ -165,6 +217,12 @@ class PipelineExecutor {
   std::vector<uintC> total_counts_;
   std::vector<std::vector<LevelStat>> level_stats_;
   std::vector<double> match_ms_;
+  // prefix sharing: the plans that reached the current level through the
+  // same levels (plan_ is the first of them)
+  bool sharing_ = false;
+  std::vector<int> group_;
+  double shared_ms_ = 0;
+  uint64_t shared_runs_ = 0;
 };
 
 }  // namespace vsgm

@@ -74,6 +74,9 @@ int Run(int argc, char** argv) {
   const int gpu_k_iters = fl.i("-gki", 10);
   const int gpu_k_seed = fl.i("-gks", 1);
   const int device_views = fl.i("-dv", 0);
+  // -sp 1 matches the plans as a prefix tree (levels several queries reach
+  // identically run once for all of them; e.g. the -km graphlets)
+  const int share_prefixes = fl.i("-sp", 0);
   // -gp P packs the k-means clusters into bins of at most -m GB of list
   // entries with P bins open per cluster (the view_packing tool's step).
   const int gpu_pack = fl.i("-gp", 0);
@@ -262,16 +265,25 @@ int Run(int argc, char** argv) {
   for (int c = 0; c < consumers; ++c)
     threads.emplace_back([&, c] {
       try {
-        PipelineExecutor exec(c, graph.vertex_count, plan_ptrs, max_sources);
+        ExecutorOptions eo;
+        eo.share_prefixes = share_prefixes != 0;
+        PipelineExecutor exec(c, graph.vertex_count, plan_ptrs, max_sources, eo);
         if (device_views) {
           exec.SetGraph(graph);
-          for (int bin; (bin = next_bin++) < (int)pool.size() && !failed;) {
-            if (done.count(bin)) continue;
+          // the next bin's view is built in the background while this one matches
+          auto take = [&] {
+            for (int bin; (bin = next_bin++) < (int)pool.size() && !failed;)
+              if (!done.count(bin)) return bin;
+            return -1;
+          };
+          for (int bin = take(); bin >= 0;) {
             auto t = std::chrono::steady_clock::now();
             exec.TransferDevice(pool[bin]->GetSources(), hop, root_degree);
             std::ostringstream os;
             os << "consumer " << c << " builds view " << bin << " (" << Ms(t) << " ms)\n";
             say(os.str());
+            const int next = take();
+            if (next >= 0) exec.PrefetchDeviceView(pool[next]->GetSources(), hop, root_degree);
             t = std::chrono::steady_clock::now();
             const std::vector<uintC> before = exec.GetTotalCounts();
             exec.Match(do_match != 0);
@@ -279,6 +291,7 @@ int Run(int argc, char** argv) {
             std::ostringstream os2;
             os2 << "consumer " << c << " completes match of view " << bin << " (" << Ms(t) << " ms)\n";
             say(os2.str());
+            bin = next;
           }
           counts[c] = exec.GetTotalCounts();
           return;

@@ -89,6 +89,8 @@ def _declare(L):
         "vsgm_ctx_sync": ([P], i),
         "vsgm_last_error": ([P], C.c_char_p),
         "vsgm_mem_info": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], i),
+        "vsgm_ctx_set_budget": ([P, u64], i),
+        "vsgm_ctx_bind_thread": ([P], i),
         "vsgm_alloc": ([P, u64, C.POINTER(P)], i),
         "vsgm_free": ([P, P], i),
         "vsgm_h2d": ([P, P, P, u64], i),
