@@ -152,6 +152,34 @@ def schedule_steps(run_first, run_step, restart, warmup, steps, budget_s, now, r
     return warm, timed, t0, now()
 
 
+def golden_check(args, g, keys, counts):
+    """Counts of the whole run against the committed golden counts of the
+    same graph (tests/golden/rmat_counts.json, written by the oracle's
+    textbook counters; the graph is identified by its CSR digest). Returns
+    the comparison, or None when no golden record covers this graph; a
+    mismatch stops the run."""
+    path = os.path.join(ROOT, "tests", "golden", "rmat_counts.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(f"rmat{args.scale}")
+    if not rec or rec["edge_factor"] != args.edge_factor or rec["seed"] != args.seed:
+        return None
+    import numpy as np
+
+    rp, cl = g.csr(copy=False)
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(rp, dtype=np.uint64).tobytes())
+    h.update(np.ascontiguousarray(cl, dtype=np.uint32).tobytes())
+    if h.hexdigest() != rec["csr_sha256"]:
+        return None
+    have = {k: c for k, c in zip(keys, counts) if k in rec["counts"]}
+    out = {"checked": sorted(have), "equal": all(rec["counts"][k] == c for k, c in have.items())}
+    log(f"golden counts: {out}")
+    assert out["equal"], f"counts {have} differ from the golden counts {rec['counts']}"
+    return out
+
+
 def load_traffic(kernel, workload, plan, isect_kernel):
     """Measured traffic of `kernel` on this exact configuration, or None: the
     profile file must name the same workload, plan, map kernel variant and
@@ -189,7 +217,7 @@ def main():
     ap.add_argument("--cl-vertices", type=int, default=65_600_000)
     ap.add_argument("--cl-edges", type=float, default=1.81e9, help="target undirected edges of the Chung-Lu graph")
     ap.add_argument("--cl-sample-s", type=float, default=120.0, help="Chung-Lu: seconds a sampled 4-clique / house pass may grow to")
-    ap.add_argument("--cl-parity-roots", type=int, default=16, help="Chung-Lu: roots whose counts are checked against the oracle")
+    ap.add_argument("--cl-parity-roots", type=int, default=8, help="Chung-Lu: roots whose counts are checked against the oracle")
     ap.add_argument("--workload", choices=["single", "batched", "copy-calibration", "chunglu"], default="single",
                     help="single: the 4-clique/diamond/4-cycle line; batched: many random patterns over k view bins, "
                          "one-at-a-time vs batched+tuned vs learned+batched; copy-calibration: kernels of known HBM "
@@ -198,6 +226,7 @@ def main():
                     help="matching orders of the single workload: the heuristic planner (the metric's configuration) "
                          "or the learned planner")
     ap.add_argument("--n-patterns", type=int, default=64)
+    ap.add_argument("--share-prefixes", type=int, default=1, help="batched: match the plans as a prefix tree")
     ap.add_argument("--bins", type=int, default=8)
     args = ap.parse_args()
     os.environ["VSGM_PROGRESS"] = str(args.progress)
@@ -309,6 +338,7 @@ def main():
     assert all(c * steps == tc for c, tc in zip(counts, ex.total_counts)), "counts differ between steps"
     if ref_counts is not None:
         assert counts == ref_counts, f"timed counts {counts} differ from warmup counts {ref_counts}"
+    golden = golden_check(args, g, keys, counts)
     ms_per_step = (t1 - t0) * 1000.0 / steps
     match_ms = [m / steps for m in ex.match_ms]
     value = sum(counts) / (ms_per_step / 1000.0)
@@ -369,6 +399,7 @@ def main():
             "cold_weight": args.cold_weight,
             "ticket_chunks": args.ticket_chunks,
             "sampled_root_parity": sample_check,
+            "golden_counts": golden,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -436,12 +467,16 @@ def learned_plans(V, args, keys, queries, g):
                           seed=args.seed + 1)
     big["group"] = big["group"] + int(data["group"].max()) + 1
     data = {k: np.concatenate([data[k], big[k]]) for k in data}
-    model, losses = learned.train(data, epochs=60, batch=128, lr=2e-3, seed=args.seed)
+    # a fifth of the record groups is held out to set the margin
+    train_set, held = learned.split_groups(data, holdout=0.2, seed=args.seed)
+    model, losses = learned.train(train_set, epochs=60, batch=128, lr=2e-3, seed=args.seed)
+    margin, table = learned.calibrate_margin(model, held)
     rp_all, _ = g.csr(copy=False)
     vstats = learned.view_stats(rp_all, g.vertex_count)
-    planner = learned.LearnedPlanner(model, margin=0.1)
+    planner = learned.LearnedPlanner(model, margin=margin)
     chosen = [planner.plan(q, vstats) for q in queries]
-    info = {"records": int(len(data["y"])), "loss": [float(losses[0]), float(losses[-1])],
+    info = {"records": int(len(data["y"])), "train_records": int(len(train_set["y"])), "margin": margin,
+            "margin_table": table, "loss": [float(losses[0]), float(losses[-1])],
             "orders": {k: (c.get("order") if c["learned"] else "heuristic") for k, (_, c) in zip(keys, chosen)},
             "predicted_gain": {k: (round(c["pred_gain"], 4), c.get("best_order", c.get("order")))
                                for k, (_, c) in zip(keys, chosen)},
@@ -464,14 +499,17 @@ def chunglu_main(args):
       views      every bin's view built on the device from the resident
                  graph, the next bin's build overlapped with the current
                  bin's match (a second context and stream);
-      patterns   triangle over every source of every 2-hop bin, and again
-                 over the 3-hop bins; 4-clique and house over a seeded
-                 sample of each 2-hop bin's sources that grows 4x while a
-                 pass takes under an eighth of --cl-sample-s (labelled with
-                 its fraction).
-    Counts: the 2-hop and 3-hop triangle totals equal the same plan over one
-    whole-graph view, and all three patterns equal the oracle's on a seeded
-    sample of root vertices."""
+      patterns   triangle over every source of every 2-hop bin (and over the
+                 3-hop bins when the packing succeeds; where the reference's
+                 packer refuses 3-hop bins -m 10, one 2-hop bin's sources
+                 are also matched over their 3-hop device view); 4-clique
+                 and house over a seeded sample of each 2-hop bin's sources,
+                 starting at 1/4096 of them (4-clique) or 4 per bin (house)
+                 and growing 4x while a pass takes under an eighth of
+                 --cl-sample-s (labelled with the sample).
+    Counts: the triangle totals equal the same plan over one whole-graph
+    view, and all three patterns equal the oracle's on a seeded sample of
+    root vertices."""
     import numpy as np
 
     import oracle as O
@@ -549,21 +587,31 @@ def chunglu_main(args):
                                    "view_wait_s": round(build, 3), "embeddings_per_s": counts[0] / match}
         if h == 2:
             kstats_main = kst
+    nb, bin_of = packs[2]
+    if 3 not in packs:
+        # the smallest 2-hop bin's sources over their 3-hop device view: same triangles as over the 2-hop one
+        small = int(np.argmin(np.bincount(bin_of, minlength=nb)))
+        srcs = np.nonzero(bin_of == small)[0].astype(np.uint32)
+        c2, _, m2, _, _ = run_bins([plans["kc3"]], [srcs], 2, f"triangle, bin {small} ({len(srcs)} sources), 2-hop view")
+        c3, _, m3, b3, _ = run_bins([plans["kc3"]], [srcs], 3, f"triangle, bin {small} ({len(srcs)} sources), 3-hop view")
+        assert c2 == c3, f"3-hop view counts {c3}, 2-hop view {c2}"
+        out["triangle_3hop_view_of_one_bin"] = {"bin": small, "sources": int(len(srcs)), "count": c3[0],
+                                                "match_s": round(m3, 3), "view_s": round(b3, 3)}
     # 4-clique and house over growing samples of every 2-hop bin's sources
     rng = np.random.default_rng(args.seed)
-    nb, bin_of = packs[2]
     order = [rng.permutation(np.nonzero(bin_of == b)[0].astype(np.uint32)) for b in range(nb)]
-    for k in ("kc4", "p4"):
-        frac = 1.0 / 4096
+    for k, first in (("kc4", lambda o: max(1, len(o) // 4096)), ("p4", lambda o: min(4, len(o)))):
+        take = [first(o) for o in order]
         best = None
         while True:
-            sample = [np.sort(o[: max(1, int(len(o) * frac))]) for o in order]
-            counts, wall, match, build, _ = run_bins([plans[k]], sample, plans[k].hop, f"{NAMES[k]}, {frac:.6f} of the sources")
-            best = {"count": counts[0], "sample_fraction": frac, "sources": int(sum(len(x) for x in sample)),
+            sample = [np.sort(o[:n]) for o, n in zip(order, take)]
+            n_src = int(sum(len(x) for x in sample))
+            counts, wall, match, build, _ = run_bins([plans[k]], sample, plans[k].hop, f"{NAMES[k]}, {n_src} sampled sources")
+            best = {"count": counts[0], "sample_sources": n_src, "sample_fraction": n_src / g.vertex_count,
                     "seconds": round(wall, 3), "match_s": round(match, 3), "embeddings_per_s": counts[0] / match if match else 0.0}
-            if frac >= 1.0 or wall * 4 > args.cl_sample_s / 2 or elapsed() + wall * 4 > args.budget_s:
+            if n_src >= g.vertex_count or wall * 4 > args.cl_sample_s / 2 or elapsed() + wall * 4 > args.budget_s:
                 break
-            frac = min(1.0, frac * 4)
+            take = [min(len(o), 4 * n) for o, n in zip(order, take)]
         out[NAMES[k]] = best
     # parity: all three plans against the oracle on seeded roots
     roots = np.sort(rng.permutation(g.vertex_count)[: args.cl_parity_roots]).astype(np.uint32)
@@ -624,8 +672,10 @@ def batched_main(args):
                (host-built view of each bin transferred, then Match one plan
                after another);
       batched  all bins' views built on the device and resident, every plan
-               autotuned, the whole enumeration captured once into one
-               device graph and replayed;
+               autotuned, the plans run as a prefix tree (levels several
+               patterns reach identically run once, --share-prefixes), the
+               whole enumeration captured once into one device graph and
+               replayed;
       learned  as batched, with matching orders from the learned planner
                (trained here on records from smaller seeded R-MAT graphs).
     Every count is checked equal across the three. Times are the match phase
@@ -655,8 +705,11 @@ def batched_main(args):
     tq += [V.Query.from_matrix(six[i][2]) for i in np.random.default_rng(args.seed + 1).choice(len(six), 24, replace=False)]
     data = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (8, 9)], tq, bins=2, max_orders=12, repeats=1,
                            seed=args.seed)
-    model, losses = learned.train(data, epochs=60, batch=128, lr=2e-3, seed=args.seed)
-    log(f"learned planner: {len(data['y'])} records, loss {losses[0]:.3f} -> {losses[-1]:.3f} in {time.perf_counter() - t:.1f}s")
+    train_set, held = learned.split_groups(data, holdout=0.2, seed=args.seed)
+    model, losses = learned.train(train_set, epochs=60, batch=128, lr=2e-3, seed=args.seed)
+    margin, _ = learned.calibrate_margin(model, held)
+    log(f"learned planner: {len(data['y'])} records, loss {losses[0]:.3f} -> {losses[-1]:.3f}, held-out margin {margin} "
+        f"in {time.perf_counter() - t:.1f}s")
 
     # the bins: k-means on the GPU over closed 1-hop neighbourhoods
     from vsgm_amd import partition
@@ -677,7 +730,7 @@ def batched_main(args):
             holders.append(h)
         return vbm, holders
 
-    planner = learned.LearnedPlanner(model, margin=0.1)
+    planner = learned.LearnedPlanner(model, margin=margin)
     vbm0, holders0 = make_bins(heur)
     rp0, _, src0 = holders0[0].csr(g.vertex_count)
     vstats = learned.view_stats(rp0, len(src0))
@@ -723,6 +776,7 @@ def batched_main(args):
         del tx
         bx = V.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
         bx.set_option("batch_bytes", 2 << 30)
+        bx.set_option("share_prefixes", int(args.share_prefixes))
         bx.set_graph(g)
         t = time.perf_counter()
         for s in bin_sources:

@@ -9,6 +9,13 @@
  * even of FP32 master copies); every product is accumulated in FP32; biases
  * are FP32. Loss: mean over the batch of (y - t)^2 / 2.
  *
+ * Two kernel paths (vsgm_mlp_set_mfma): the default runs every matrix
+ * product on the matrix cores (bf16 v_mfma_f32_32x32x8_bf16, FP32
+ * accumulation), with the products' activations and backward deltas
+ * rounded to BF16 as well (x, relu(z1), dy w3 [z2 > 0], (dz2 W2) [h1 > 0]);
+ * the other keeps activations in FP32 (one wavefront per sample, scalar
+ * FMAs).
+ *
  * Parameters are one FP32 vector in this order:
  *   W1 [64][in_dim] | b1 [64] | W2 [64][64] | b2 [64] | w3 [64] | b3 [1].
  * Training is Adam on the FP32 masters; after each step the BF16 copies are
@@ -43,6 +50,8 @@ int vsgm_mlp_forward(vsgm_mlp* m, const float* d_x, uint64_t batch, float* d_y);
  * through the BF16 weights), kept inside the model; *h_loss gets the loss.
  * With h_grad non-NULL the gradient is also copied to the host (n floats). */
 int vsgm_mlp_backward(vsgm_mlp* m, const float* d_x, const float* d_t, uint64_t batch, float* h_loss, float* h_grad, uint64_t n);
+/* 1 (default): matrix-core kernels; 0: FP32-activation kernels. */
+int vsgm_mlp_set_mfma(vsgm_mlp* m, int on);
 /* One Adam step with the last gradient (t counts steps from 1). */
 int vsgm_mlp_adam_step(vsgm_mlp* m, float lr, float beta1, float beta2, float eps);
 

@@ -10,6 +10,11 @@ definition of round-to-nearest-even BF16.
 Model: y = w3q . relu(W2q relu(W1q x + b1) + b2) + b3 with Wq = bf16(W)
 (the gradient flows through the rounding unchanged); loss = mean (y-t)^2/2.
 Parameters: W1 [64][D] | b1 [64] | W2 [64][64] | b2 [64] | w3 [64] | b3 [1].
+
+mfma=True restates the matrix-core kernels: the operands of every product
+are BF16, so x and h1 = relu(z1) are rounded before the forward products,
+and the deltas dh2 = dy w3 [z2 > 0] and dh1 = (bf16(dh2) W2) [h1 > 0]
+before the backward ones (bias and w3 gradients, h2 and y stay FP32).
 """
 import numpy as np
 
@@ -38,23 +43,25 @@ def unpack(p: np.ndarray, d: int):
     return W1, b1, W2, b2, w3, b3
 
 
-def forward(p: np.ndarray, x: np.ndarray, quantize: bool = True, dtype=np.float32):
+def forward(p: np.ndarray, x: np.ndarray, quantize: bool = True, dtype=np.float32, mfma: bool = False):
     d = x.shape[1]
     W1, b1, W2, b2, w3, b3 = unpack(np.asarray(p, dtype=np.float32 if quantize else dtype), d)
     q = bf16 if quantize else (lambda a: a)
     W1, W2, w3 = (q(W1).astype(dtype), q(W2).astype(dtype), q(w3).astype(dtype))
     b1, b2, b3 = b1.astype(dtype), b2.astype(dtype), dtype(b3)
-    x = x.astype(dtype)
+    x = bf16(x) if mfma else x.astype(dtype)
     z1 = x @ W1.T + b1
     h1 = np.maximum(z1, 0)
+    if mfma:
+        h1 = bf16(h1)
     z2 = h1 @ W2.T + b2
     h2 = np.maximum(z2, 0)
     y = h2 @ w3 + b3
     return y, (x, z1, h1, z2, h2, W1, W2, w3)
 
 
-def loss_and_grad(p: np.ndarray, x: np.ndarray, t: np.ndarray, quantize: bool = True, dtype=np.float32):
-    y, (xx, z1, h1, z2, h2, W1, W2, w3) = forward(p, x, quantize, dtype)
+def loss_and_grad(p: np.ndarray, x: np.ndarray, t: np.ndarray, quantize: bool = True, dtype=np.float32, mfma: bool = False):
+    y, (xx, z1, h1, z2, h2, W1, W2, w3) = forward(p, x, quantize, dtype, mfma)
     n = len(x)
     err = y - t.astype(dtype)
     loss = 0.5 * np.sum(err * err) / n
@@ -62,11 +69,13 @@ def loss_and_grad(p: np.ndarray, x: np.ndarray, t: np.ndarray, quantize: bool = 
     gw3 = h2.T @ dy
     gb3 = dy.sum()
     dh2 = np.outer(dy, w3) * (z2 > 0)
-    gW2 = dh2.T @ h1
     gb2 = dh2.sum(0)
+    if mfma:
+        dh2 = bf16(dh2)
+    gW2 = dh2.T @ h1
     dh1 = (dh2 @ W2) * (z1 > 0)
-    gW1 = dh1.T @ xx
     gb1 = dh1.sum(0)
+    gW1 = (bf16(dh1) if mfma else dh1).T @ xx
     g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gw3, [gb3]]).astype(dtype)
     return float(loss), g
 

@@ -22,11 +22,13 @@ def ctx(gpu):
     c.close()
 
 
+@pytest.mark.parametrize("mfma", [True, False])
 @pytest.mark.parametrize("d,n", [(1, 1), (17, 5), (100, 1000), (128, 333)])
-def test_forward_matches_checker(ctx, d, n):
+def test_forward_matches_checker(ctx, d, n, mfma):
     from vsgm_amd.mlp import CostModel
 
     m = CostModel(d, seed=d, ctx=ctx)
+    m.use_mfma(mfma)
     rng = np.random.default_rng(d)
     p = m.get_params()
     assert p.size == M.n_params(d)
@@ -36,16 +38,18 @@ def test_forward_matches_checker(ctx, d, n):
     assert np.array_equal(m.get_params(), p)
     x = rng.normal(0, 1, (n, d)).astype(np.float32)
     y = m.forward(x)
-    want, _ = M.forward(p, x)
+    want, _ = M.forward(p, x, mfma=mfma)
     assert np.allclose(y, want, rtol=1e-5, atol=1e-5), np.max(np.abs(y - want))
     m.close()
 
 
+@pytest.mark.parametrize("mfma", [True, False])
 @pytest.mark.parametrize("d,n", [(8, 3), (100, 777), (128, 4096)])
-def test_backward_matches_checker(ctx, d, n):
+def test_backward_matches_checker(ctx, d, n, mfma):
     from vsgm_amd.mlp import CostModel
 
     m = CostModel(d, seed=3 + d, ctx=ctx)
+    m.use_mfma(mfma)
     rng = np.random.default_rng(5)
     p = m.get_params()
     p[M.H * d:M.H * d + M.H] = rng.normal(0, 0.2, M.H)
@@ -53,7 +57,7 @@ def test_backward_matches_checker(ctx, d, n):
     x = rng.normal(0, 1, (n, d)).astype(np.float32)
     t = rng.normal(0, 1, n).astype(np.float32)
     loss, g = m.backward(x, t)
-    want_loss, want_g = M.loss_and_grad(p, x, t)
+    want_loss, want_g = M.loss_and_grad(p, x, t, mfma=mfma)
     assert abs(loss - want_loss) <= 1e-5 * abs(want_loss) + 1e-6
     scale = np.max(np.abs(want_g))
     assert np.allclose(g, want_g, rtol=1e-4, atol=1e-5 * scale), np.max(np.abs(g - want_g))

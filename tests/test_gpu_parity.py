@@ -218,6 +218,48 @@ def test_full_size_triangles_and_cliques(vsgm, oracle, gpu):
     assert run_gpu(vsgm, g, [vsgm.Plan.optimize(vsgm.Query.clique(4))]) == [oracle.four_cliques(rp, cl)]
 
 
+def _golden(vsgm, scale):
+    """Golden counts of R-MAT scale `scale` (tests/golden/make_rmat_counts.py)
+    and the graph they were computed on (its CSR digest checked)."""
+    import hashlib
+    import json
+
+    rec = json.load(open(os.path.join(ROOT, "tests", "golden", "rmat_counts.json")))[f"rmat{scale}"]
+    g = vsgm.Graph.rmat(scale, rec["edge_factor"], *rec["abc"], seed=rec["seed"])
+    rp, cl = g.csr(copy=False)
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(rp, dtype=np.uint64).tobytes())
+    h.update(np.ascontiguousarray(cl, dtype=np.uint32).tobytes())
+    assert h.hexdigest() == rec["csr_sha256"], "the generator no longer makes the graph of the golden counts"
+    return g, rec["counts"]
+
+
+def test_benchmark_graph_counts_against_golden(vsgm, gpu):
+    """The benchmark pipeline (k-means with k = 1, 2-hop view with the
+    root-degree filter, heuristic plans) on R-MAT scale 22: 4-cliques and
+    diamonds at full size, and all three benchmark patterns at scale 20,
+    equal the golden counts of the oracle's textbook counters (the scale-22
+    4-cycle, ~200 s on the GPU, is checked by every bench run)."""
+    from vsgm_amd import partition
+
+    for scale, keys in ((22, ("kc4", "p2")), (20, ("kc4", "p2", "p1"))):
+        g, want = _golden(vsgm, scale)
+        qs = {"kc4": vsgm.Query.clique(4), "p2": vsgm.Query.preset(2), "p1": vsgm.Query.preset(1)}
+        plans = [vsgm.Plan.optimize(qs[k]) for k in keys]
+        hop = max(p.hop for p in plans)
+        rd = min(p.root_degree for p in plans)
+        assign, _, _ = partition.kmeans(g, 1, iters=1, seed=1)
+        vbm = vsgm.ViewBinManager.from_assignment(g, hop, 1, assign)
+        holder = vsgm.ViewBinHolder(g.vertex_count, max(4, g.entries * 4))
+        vbm.materialize(0, holder, rd)
+        ex = vsgm.PipelineExecutor(0, g.vertex_count, plans, max_sources=vbm.max_sources)
+        ex.transfer(holder)
+        print(f"[gpu] R-MAT s{scale} {keys} against the golden counts", flush=True)
+        ex.match()
+        assert ex.total_counts == [want[k] for k in keys], (scale, ex.total_counts)
+        del ex, holder, vbm, g
+
+
 def test_pm_pipeline_cli(vsgm, oracle, gpu, tmp_path):
     g = vsgm.Graph.rmat(10, 16, seed=12)
     path = str(tmp_path / "rmat10.bin")

@@ -165,6 +165,195 @@ __global__ __launch_bounds__(kBlock) void k_mlp(const float* __restrict__ x, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core path: every product of the forward and backward passes as
+// bf16 MFMA tiles with FP32 accumulation (v_mfma_f32_32x32x8_bf16). A
+// workgroup takes tiles of 64 samples; each of its 4 wavefronts computes one
+// 32 x 32 block of every 64 x 64 product (and two blocks of the 64 x D
+// weight gradient), keeping the weight-gradient blocks in accumulator
+// registers across tiles. Activations and backward deltas enter the
+// products rounded to BF16 (the sums and the bias / w3 terms stay FP32):
+// x, h1 = bf16(relu(z1)), dz2 = bf16(dy w3 [z2 > 0]), dz1 = bf16((dz2 W2)
+// [h1 > 0]).
+//
+// Operand layout of v_mfma_f32_32x32x8_bf16 (A 32 x 8, B 8 x 32): lane l
+// supplies A[l % 32][4 (l / 32) + i] and B[4 (l / 32) + i][l % 32], i < 4,
+// and holds C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32] in accumulator r < 16.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kT = 64;               // samples per tile
+constexpr int kXS = kMaxIn + 8;      // LDS row stride (elements) of x / W1 tiles
+constexpr int kHS = H + 8;           // LDS row stride of 64-wide BF16 tiles
+constexpr int kH2S = H + 1;          // LDS row stride of the FP32 tile
+
+__device__ inline bf16x4 row4(const uint16_t* p) {  // p[0..3], 8-byte aligned
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  bf16x4 r;
+  r[0] = (short)(v.x & 0xFFFFu);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xFFFFu);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+__device__ inline bf16x4 col4(const uint16_t* p, int stride) {  // p[0], p[stride], p[2 stride], p[3 stride]
+  bf16x4 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) r[i] = (short)p[i * stride];
+  return r;
+}
+
+// acc (32 x 32) += A (32 x K) B (K x 32). Operand access: A by rows (a[i][k]
+// at a + i * lda + k) or by columns (a[i][k] at a + k * lda + i); likewise B
+// by rows of its transpose (b[k][j] at b + j * ldb + k) or directly
+// (b[k][j] at b + k * ldb + j).
+template <bool A_ROWS, bool B_T>
+__device__ inline void mma(f32x16& acc, const uint16_t* a, int lda, const uint16_t* b, int ldb, int K, int lane) {
+  const int i = lane & 31, kq = (lane >> 5) * 4;
+  for (int k = 0; k < K; k += 8) {
+    const bf16x4 av = A_ROWS ? row4(a + i * lda + k + kq) : col4(a + (k + kq) * lda + i, lda);
+    const bf16x4 bv = B_T ? row4(b + i * ldb + k + kq) : col4(b + (k + kq) * ldb + i, ldb);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+}
+
+__device__ inline int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+__global__ __launch_bounds__(kBlock) void k_mlp_mfma(const float* __restrict__ x, const float* __restrict__ t, uint64_t batch, int D,
+                                                     const uint16_t* __restrict__ wq, const float* __restrict__ params,
+                                                     float* __restrict__ y, float* __restrict__ grad, float* __restrict__ loss,
+                                                     float scale) {
+  __shared__ __align__(16) uint16_t s_w1[H * kXS];   // W1 [j][d], zero beyond D
+  __shared__ __align__(16) uint16_t s_w2[H * kHS];   // W2 [j][k]
+  __shared__ __align__(16) uint16_t s_x[kT * kXS];   // x tile [s][d]
+  __shared__ __align__(16) uint16_t s_h1[kT * kHS];  // h1 tile [s][k]
+  __shared__ __align__(16) uint16_t s_dz2[kT * kHS];
+  __shared__ __align__(16) uint16_t s_dz1[kT * kHS];
+  __shared__ float s_f[kT * kH2S];                   // h2 tile, then dz1 (FP32)
+  __shared__ float s_dy[kT];
+  __shared__ float s_b1[H], s_b2[H], s_w3[H];
+  const Layout L(D);
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const bool backward = grad != nullptr;
+  const int Dp = (D + 7) & ~7;
+  for (int i = tid; i < H * kXS; i += kBlock) {
+    const int j = i / kXS, d = i % kXS;
+    s_w1[i] = d < D ? wq[j * D + d] : (uint16_t)0;
+  }
+  for (int i = tid; i < H * kHS; i += kBlock) {
+    const int j = i / kHS, k = i % kHS;
+    s_w2[i] = k < H ? wq[H * D + j * H + k] : (uint16_t)0;
+  }
+  if (tid < H) {
+    s_b1[tid] = params[L.b1 + tid];
+    s_b2[tid] = params[L.b2 + tid];
+    s_w3[tid] = from_bf16(wq[H * D + H * H + tid]);
+  }
+  const float b3 = params[L.b3];
+  const int rb = (wave >> 1) * 32, cb = (wave & 1) * 32;  // this wave's block of a 64 x 64 product
+  f32x16 g2 = {0}, g1a = {0}, g1b = {0};                   // dW2 block (rb, cb); dW1 blocks (rb, cb) and (rb, cb + 64)
+  float gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, lsum = 0.f;
+  __syncthreads();
+  for (uint64_t base = (uint64_t)blockIdx.x * kT; base < batch; base += (uint64_t)gridDim.x * kT) {
+    for (int i = tid; i < kT * kXS; i += kBlock) {
+      const int s = i / kXS, d = i % kXS;
+      const uint64_t g = base + s;
+      s_x[i] = (g < batch && d < D) ? to_bf16(x[g * D + d]) : (uint16_t)0;
+    }
+    __syncthreads();
+    f32x16 acc = {0};
+    mma<true, true>(acc, s_x + rb * kXS, kXS, s_w1 + cb * kXS, kXS, Dp, lane);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int s = rb + acc_row(r, lane), k = cb + (lane & 31);
+      s_h1[s * kHS + k] = to_bf16(fmaxf(acc[r] + s_b1[k], 0.f));
+    }
+    __syncthreads();
+    acc = f32x16{0};
+    mma<true, true>(acc, s_h1 + rb * kHS, kHS, s_w2 + cb * kHS, kHS, H, lane);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int s = rb + acc_row(r, lane), j = cb + (lane & 31);
+      s_f[s * kH2S + j] = fmaxf(acc[r] + s_b2[j], 0.f);
+    }
+    __syncthreads();
+    if (tid < kT) {
+      float yv = b3;
+      for (int j = 0; j < H; ++j) yv += s_f[tid * kH2S + j] * s_w3[j];
+      const uint64_t g = base + tid;
+      float dy = 0.f;
+      if (g < batch) {
+        if (y) y[g] = yv;
+        if (backward) {
+          const float err = yv - t[g];
+          dy = err * scale;
+          lsum += 0.5f * err * err * scale;
+        }
+      }
+      s_dy[tid] = dy;
+    }
+    if (!backward) {
+      __syncthreads();
+      continue;
+    }
+    __syncthreads();
+    for (int i = tid; i < kT * H; i += kBlock) {
+      const int s = i / H, j = i % H;
+      s_dz2[s * kHS + j] = to_bf16(s_f[s * kH2S + j] > 0.f ? s_dy[s] * s_w3[j] : 0.f);
+    }
+    if (tid < H) {
+      float a = 0.f, b = 0.f;
+      for (int s = 0; s < kT; ++s) {
+        const float h2 = s_f[s * kH2S + tid];
+        a += s_dy[s] * h2;
+        b += h2 > 0.f ? s_dy[s] * s_w3[tid] : 0.f;
+      }
+      gw3 += a;
+      gb2 += b;
+    }
+    if (tid == 0)
+      for (int s = 0; s < kT; ++s) gb3 += s_dy[s];
+    __syncthreads();
+    // dW2 += dz2^T h1 ; dh1 = dz2 W2
+    mma<false, false>(g2, s_dz2 + rb, kHS, s_h1 + cb, kHS, kT, lane);
+    acc = f32x16{0};
+    mma<true, false>(acc, s_dz2 + rb * kHS, kHS, s_w2 + cb, kHS, H, lane);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int s = rb + acc_row(r, lane), k = cb + (lane & 31);
+      const float v = from_bf16(s_h1[s * kHS + k]) > 0.f ? acc[r] : 0.f;
+      s_f[s * kH2S + k] = v;
+      s_dz1[s * kHS + k] = to_bf16(v);
+    }
+    __syncthreads();
+    if (tid < H)
+      for (int s = 0; s < kT; ++s) gb1 += s_f[s * kH2S + tid];
+    // dW1 += dz1^T x
+    mma<false, false>(g1a, s_dz1 + rb, kHS, s_x + cb, kXS, kT, lane);
+    if (Dp > 64) mma<false, false>(g1b, s_dz1 + rb, kHS, s_x + cb + 64, kXS, kT, lane);
+    __syncthreads();  // the next tile overwrites every tile buffer
+  }
+  if (!backward) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int j = rb + acc_row(r, lane), k = cb + (lane & 31);
+    if (g2[r] != 0.f) atomicAdd(&grad[L.w2 + j * H + k], g2[r]);
+    const int d0 = cb + (lane & 31), d1 = d0 + 64;
+    if (d0 < D && g1a[r] != 0.f) atomicAdd(&grad[L.w1 + j * D + d0], g1a[r]);
+    if (d1 < D && g1b[r] != 0.f) atomicAdd(&grad[L.w1 + j * D + d1], g1b[r]);
+  }
+  if (tid < H) {
+    atomicAdd(&grad[L.b1 + tid], gb1);
+    atomicAdd(&grad[L.b2 + tid], gb2);
+    atomicAdd(&grad[L.w3 + tid], gw3);
+  }
+  if (tid == 0) {
+    atomicAdd(&grad[L.b3], gb3);
+  }
+  if (tid < kT) atomicAdd(loss, lsum);
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero_f32(float* d, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) d[i] = 0.f;
 }
@@ -211,6 +400,7 @@ unsigned grid_for(uint64_t units, unsigned cap = 1024) { return (unsigned)std::m
 
 struct vsgm_mlp {
   vsgm_ctx* ctx = nullptr;
+  bool mfma = true;  // matrix-core kernels (else one wavefront per sample, FP32 activations)
   int D = 0;
   uint64_t n = 0;
   float* params = nullptr;
@@ -316,8 +506,12 @@ int vsgm_mlp_set_params(vsgm_mlp* m, const float* h_params, uint64_t n) {
 int vsgm_mlp_forward(vsgm_mlp* m, const float* d_x, uint64_t batch, float* d_y) {
   if (!m || (batch && (!d_x || !d_y))) return VSGM_ERR_ARG;
   if (batch == 0) return VSGM_OK;
-  hipLaunchKernelGGL(k_mlp, dim3(grid_for((batch + kWaves - 1) / kWaves)), dim3(kBlock), 0, vsgm_internal::stream(m->ctx), d_x,
-                     (const float*)nullptr, batch, m->D, m->wq, m->params, d_y, (float*)nullptr, (float*)nullptr, 0.f);
+  if (m->mfma)
+    hipLaunchKernelGGL(k_mlp_mfma, dim3(grid_for((batch + kT - 1) / kT)), dim3(kBlock), 0, vsgm_internal::stream(m->ctx), d_x,
+                       (const float*)nullptr, batch, m->D, m->wq, m->params, d_y, (float*)nullptr, (float*)nullptr, 0.f);
+  else
+    hipLaunchKernelGGL(k_mlp, dim3(grid_for((batch + kWaves - 1) / kWaves)), dim3(kBlock), 0, vsgm_internal::stream(m->ctx), d_x,
+                       (const float*)nullptr, batch, m->D, m->wq, m->params, d_y, (float*)nullptr, (float*)nullptr, 0.f);
   return launch_ok(m->ctx, "k_mlp launch failed");
 }
 
@@ -327,12 +521,22 @@ int vsgm_mlp_backward(vsgm_mlp* m, const float* d_x, const float* d_t, uint64_t 
   int s = zero(m, m->grad, m->n);
   if (s == VSGM_OK) s = zero(m, m->loss, 1);
   if (s != VSGM_OK) return s;
-  // at most 256 workgroups: each adds its LDS gradient once
-  hipLaunchKernelGGL(k_mlp, dim3(grid_for((batch + kWaves - 1) / kWaves, 256)), dim3(kBlock), 0, vsgm_internal::stream(m->ctx), d_x, d_t,
-                     batch, m->D, m->wq, m->params, (float*)nullptr, m->grad, m->loss, (float)(1.0 / (double)batch));
+  // at most 256 workgroups: each adds its gradient once
+  if (m->mfma)
+    hipLaunchKernelGGL(k_mlp_mfma, dim3(grid_for((batch + kT - 1) / kT, 256)), dim3(kBlock), 0, vsgm_internal::stream(m->ctx), d_x, d_t,
+                       batch, m->D, m->wq, m->params, (float*)nullptr, m->grad, m->loss, (float)(1.0 / (double)batch));
+  else
+    hipLaunchKernelGGL(k_mlp, dim3(grid_for((batch + kWaves - 1) / kWaves, 256)), dim3(kBlock), 0, vsgm_internal::stream(m->ctx), d_x,
+                       d_t, batch, m->D, m->wq, m->params, (float*)nullptr, m->grad, m->loss, (float)(1.0 / (double)batch));
   if ((s = launch_ok(m->ctx, "k_mlp launch failed")) != VSGM_OK) return s;
   if (h_loss && (s = vsgm_d2h(m->ctx, h_loss, m->loss, 4)) != VSGM_OK) return s;
   if (h_grad && (s = vsgm_d2h(m->ctx, h_grad, m->grad, n * 4)) != VSGM_OK) return s;
+  return VSGM_OK;
+}
+
+int vsgm_mlp_set_mfma(vsgm_mlp* m, int on) {
+  if (!m) return VSGM_ERR_ARG;
+  m->mfma = on != 0;
   return VSGM_OK;
 }
 

@@ -26,7 +26,11 @@ Features (IN_DIM = 102, all in [-1, 1] or small):
 Choice. All connected orders (up to `max_orders`, the heuristic's always
 included) are scored on the GPU in one batch; the lowest predicted relative
 cost wins only if it predicts at least `margin` (log units) of gain over the
-heuristic order, otherwise the heuristic plan is kept.
+heuristic order, otherwise the heuristic plan is kept. `calibrate_margin`
+sets the margin from record groups held out of training: the smallest
+candidate margin under which no held-out decision the model would take is
+slower than the heuristic order (the realised gain of the decisions taken
+breaks ties toward the larger one).
 """
 import itertools
 import math
@@ -146,6 +150,48 @@ def train(data: Dict[str, np.ndarray], epochs: int = 60, batch: int = 128, lr: f
     m = CostModel(IN_DIM, seed=seed, ctx=ctx)
     losses = m.train(data["x"], data["y"], epochs=epochs, batch=batch, lr=lr, seed=seed)
     return m, losses
+
+
+def split_groups(data: Dict[str, np.ndarray], holdout: float = 0.2, seed: int = 0):
+    """Records split by group (one pattern over one view): (train, held out)."""
+    groups = np.unique(data["group"])
+    rng = np.random.default_rng(seed)
+    held = set(rng.choice(groups, size=max(1, int(round(len(groups) * holdout))), replace=False).tolist())
+    mask = np.array([g in held for g in data["group"]])
+    return {k: v[~mask] for k, v in data.items()}, {k: v[mask] for k, v in data.items()}
+
+
+MARGINS = (0.0, 0.05, 0.1, 0.2, 0.3, 0.5, 0.75, 1.0, 1.5, 2.0, math.inf)
+
+
+def decisions(pred: np.ndarray, data: Dict[str, np.ndarray]):
+    """Per group: (predicted gain of its best-predicted order over the
+    heuristic order, realised gain of that order), both in log units. The
+    heuristic order is the record whose label is 0 (its own time ratio)."""
+    out = []
+    for g in np.unique(data["group"]):
+        idx = np.nonzero(data["group"] == g)[0]
+        h = idx[np.argmin(np.abs(data["y"][idx]))]
+        b = idx[np.argmin(pred[idx])]
+        out.append((float(pred[h] - pred[b]), float(-data["y"][b])))
+    return out
+
+
+def calibrate_margin(model: CostModel, held: Dict[str, np.ndarray], margins: Sequence[float] = MARGINS):
+    """The margin taken from held-out groups: the smallest one under which no
+    decision taken (predicted gain above it) is slower than the heuristic
+    order, preferring more realised gain; infinity (always the heuristic
+    plan) when every margin lets a regression through. Returns (margin,
+    per-margin table of decisions taken, regressions and realised gain)."""
+    dec = decisions(model.forward(held["x"]), held)
+    table = []
+    for m in margins:
+        taken = [(p, r) for p, r in dec if p > m]
+        table.append({"margin": m, "taken": len(taken), "regressions": sum(r < 0 for _, r in taken),
+                      "realised_gain": sum(r for _, r in taken)})
+    safe = [t for t in table if t["regressions"] == 0]
+    best = max(safe, key=lambda t: (t["realised_gain"], -t["margin"]))
+    return best["margin"], table
 
 
 class LearnedPlanner:

@@ -29,6 +29,7 @@ def _lib():
             "vsgm_mlp_forward": ([P, P, u64, P], i),
             "vsgm_mlp_backward": ([P, P, P, u64, C.POINTER(C.c_float), P, u64], i),
             "vsgm_mlp_adam_step": ([P, C.c_float, C.c_float, C.c_float, C.c_float], i),
+            "vsgm_mlp_set_mfma": ([P, i], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -60,6 +61,10 @@ class CostModel:
             self.close()
         except Exception:
             pass
+
+    def use_mfma(self, on: bool = True):
+        """Matrix-core kernels (default) or the FP32-activation kernels."""
+        self.ctx.check(_lib().vsgm_mlp_set_mfma(self.h, int(on)), "mlp mfma")
 
     # parameters --------------------------------------------------------
     def get_params(self) -> np.ndarray:
