@@ -225,6 +225,9 @@ def main():
     ap.add_argument("--plan", choices=["heuristic", "learned"], default="heuristic",
                     help="matching orders of the single workload: the heuristic planner (the metric's configuration) "
                          "or the learned planner")
+    ap.add_argument("--relabel", choices=["none", "degree"], default="none",
+                    help="vertex ids of the single workload: as generated (the metric's configuration) or in "
+                         "ascending degree order (an isomorphic graph, same counts)")
     ap.add_argument("--n-patterns", type=int, default=64)
     ap.add_argument("--share-prefixes", type=int, default=1, help="batched: match the plans as a prefix tree")
     ap.add_argument("--bins", type=int, default=8)
@@ -250,13 +253,19 @@ def main():
 
     assert V.device_count() > 0, "no GPU visible"
     keys = [k.strip() for k in args.patterns.split(",") if k.strip()]
-    workload = f"rmat{args.scale}"
+    workload = f"rmat{args.scale}" + ("" if args.relabel == "none" else f"_{args.relabel}_ids")
 
     t = time.perf_counter()
     g = V.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed)
     gen_s = time.perf_counter() - t
     log(f"R-MAT s{args.scale} ef{args.edge_factor}: V={g.vertex_count} E={g.edge_count} maxdeg={g.max_degree} "
         f"generated in {gen_s:.1f}s")
+    g_generated, relabel_s = g, None
+    if args.relabel == "degree":
+        t = time.perf_counter()
+        g, _ = g.relabel_by_degree()
+        relabel_s = time.perf_counter() - t
+        log(f"ids relabelled in ascending degree order in {relabel_s:.1f}s")
 
     queries = [make_query(V, k) for k in keys]
     plans = [V.Plan.optimize(q) for q in queries]
@@ -338,7 +347,7 @@ def main():
     assert all(c * steps == tc for c, tc in zip(counts, ex.total_counts)), "counts differ between steps"
     if ref_counts is not None:
         assert counts == ref_counts, f"timed counts {counts} differ from warmup counts {ref_counts}"
-    golden = golden_check(args, g, keys, counts)
+    golden = golden_check(args, g_generated, keys, counts)  # counts do not depend on the ids
     ms_per_step = (t1 - t0) * 1000.0 / steps
     match_ms = [m / steps for m in ex.match_ms]
     value = sum(counts) / (ms_per_step / 1000.0)
@@ -378,7 +387,8 @@ def main():
         "data": f"synthetic: seeded R-MAT (seed {args.seed}) generated on the host",
         "config": {
             "workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (.57/.19/.19/.05), k-means k=1 single-bin "
-                        f"{hop}-hop view, patterns {','.join(keys)}, {args.plan} plans",
+                        f"{hop}-hop view, patterns {','.join(keys)}, {args.plan} plans"
+                        + ("" if args.relabel == "none" else ", ids relabelled in ascending degree order"),
             "scale": args.scale,
             "vertices": g.vertex_count,
             "edges": g.edge_count,
@@ -392,6 +402,8 @@ def main():
             "budget_s": args.budget_s,
             "wall_s_at_line": round(elapsed(), 1),
             "graph_gen_s": round(gen_s, 2),
+            "relabel": args.relabel,
+            "relabel_s": None if relabel_s is None else round(relabel_s, 2),
             "view_build_s": round(view_s, 2),
             "transfer_s": round(xfer_s, 3),
             "index_lists": ex.hub_count(),
@@ -457,13 +469,18 @@ def learned_plans(V, args, keys, queries, g):
     from vsgm_amd.shapes import all_shapes
 
     t = time.perf_counter()
+
+    def rmat(s):  # record graphs carry the ids the workload uses
+        r = V.Graph.rmat(s, 16, seed=100 + s)
+        return r.relabel_by_degree()[0] if args.relabel == "degree" else r
+
     tq = [V.Query.from_matrix(s[2]) for s in all_shapes((3, 4, 5))]
-    data = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (10, 11)], tq, bins=2, max_orders=24, repeats=1,
+    data = learned.collect([rmat(s) for s in (10, 11)], tq, bins=2, max_orders=24, repeats=1,
                            seed=args.seed)
     # 4-vertex shapes also on larger graphs, where order costs separate
     # from launch overheads (scale 11 times are ~1 ms)
     q4 = [V.Query.from_matrix(s[2]) for s in all_shapes((4,))]
-    big = learned.collect([V.Graph.rmat(s, 16, seed=100 + s) for s in (13, 14)], q4, bins=2, max_orders=24, repeats=1,
+    big = learned.collect([rmat(s) for s in (13, 14)], q4, bins=2, max_orders=24, repeats=1,
                           seed=args.seed + 1)
     big["group"] = big["group"] + int(data["group"].max()) + 1
     data = {k: np.concatenate([data[k], big[k]]) for k in data}

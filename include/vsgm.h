@@ -60,6 +60,13 @@ int vsgm_graph_load(const char* path, int device, int threads, int drop_isolated
 /* Copy of g without its isolated vertices, ids renumbered in order
  * (preprocess_graph.h:67-82). */
 int vsgm_graph_drop_isolated(const vsgm_graph* g, vsgm_graph** out);
+/* Isomorphic copy of g with ids in ascending (degree, id) order; new_id
+ * (optional, vertex_count entries) receives each vertex's id in the copy.
+ * Pattern counts are unchanged; roots that must be the smallest id of an
+ * embedding then start from low-degree vertices. No counterpart in the
+ * reference (it matches ids as given): for callers whose plans root
+ * symmetric patterns at the smallest id. */
+int vsgm_graph_relabel_degree(const vsgm_graph* g, int threads, uint32_t* new_id, vsgm_graph** out);
 int vsgm_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads, vsgm_graph** out);
 int vsgm_graph_chung_lu(uint64_t vertex_count, uint64_t draws, double gamma, uint64_t seed, int threads, vsgm_graph** out);
 int vsgm_graph_info(const vsgm_graph* g, uint64_t* vertex_count, uint64_t* entries, uint64_t* max_degree);

@@ -260,6 +260,30 @@ def test_benchmark_graph_counts_against_golden(vsgm, gpu):
         del ex, holder, vbm, g
 
 
+def test_degree_relabelled_graph_counts(vsgm, oracle, gpu):
+    """Relabelling the ids in ascending degree order changes no count: every
+    preset and clique on a relabelled R-MAT graph equals the oracle on the
+    graph as generated, over the whole graph and over three view bins, and
+    the scale-20 benchmark patterns equal the golden counts."""
+    g = vsgm.Graph.rmat(10, 16, seed=27)
+    h, new_id = g.relabel_by_degree()
+    plans = [vsgm.Plan.optimize(vsgm.Query.preset(k)) for k in range(17)] + [vsgm.Plan.optimize(vsgm.Query.clique(k)) for k in (3, 4, 5)]
+    want = oracle_counts(oracle, g, plans)
+    print("[gpu] degree-relabelled R-MAT s10, presets + cliques", flush=True)
+    assert run_gpu(vsgm, h, plans) == want
+    assign = np.random.default_rng(3).integers(0, 3, size=h.vertex_count)
+    total = np.zeros(len(plans), np.int64)
+    for b in range(3):
+        total += run_gpu(vsgm, h, plans, sources=np.nonzero(assign == b)[0].astype(np.uint32))
+    assert total.tolist() == want
+    g20, gold = _golden(vsgm, 20)
+    h20, _ = g20.relabel_by_degree()
+    keys = ("kc4", "p2", "p1")
+    qs = {"kc4": vsgm.Query.clique(4), "p2": vsgm.Query.preset(2), "p1": vsgm.Query.preset(1)}
+    print("[gpu] degree-relabelled R-MAT s20 against the golden counts", flush=True)
+    assert run_gpu(vsgm, h20, [vsgm.Plan.optimize(qs[k]) for k in keys]) == [gold[k] for k in keys]
+
+
 def test_pm_pipeline_cli(vsgm, oracle, gpu, tmp_path):
     g = vsgm.Graph.rmat(10, 16, seed=12)
     path = str(tmp_path / "rmat10.bin")
@@ -267,7 +291,8 @@ def test_pm_pipeline_cli(vsgm, oracle, gpu, tmp_path):
     rng = np.random.default_rng(0)
     vsgm.ViewBinManager.write_vbmap(path + ".2.hop.vbmap", 3, rng.integers(0, 3, size=g.vertex_count))
     exe = os.path.join(ROOT, "vsgm_amd", "bin", "pm_pipeline")
-    for flags, q in ((["-kc", "4"], vsgm.Query.clique(4)), (["-p", "4"], vsgm.Query.preset(4))):
+    for flags, q in ((["-kc", "4"], vsgm.Query.clique(4)), (["-p", "4"], vsgm.Query.preset(4)),
+                     (["-p", "1", "-rl", "1"], vsgm.Query.preset(1))):  # ids relabelled, .vbmap mapped
         out = subprocess.run([exe, "-f", path, "-m", "0.1", "-qs", "2", "-pn", "2", "-cn", "1", "-t", "4"] + flags,
                              capture_output=True, text=True, timeout=120)
         assert out.returncode == 0, out.stderr[-2000:]

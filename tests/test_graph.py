@@ -148,3 +148,51 @@ def test_view_counts_sum_to_total_on_cpu(vsgm, oracle):
         vr, vc, src = holder.csr(V)
         s += oracle.count_plan(vr, vc, plan.serialize(), sources=src)
     assert s == total
+
+
+def test_relabel_by_degree_is_an_isomorphism(vsgm):
+    g = vsgm.Graph.rmat(10, 16, seed=7)
+    h, new_id = g.relabel_by_degree(threads=3)
+    assert h.validate() == ""
+    rp, cl = g.csr()
+    hr, hc = h.csr()
+    n = g.vertex_count
+    assert sorted(new_id.tolist()) == list(range(n))
+    deg, hdeg = np.diff(rp), np.diff(hr)
+    assert np.array_equal(hdeg[new_id], deg)
+    assert np.all(np.diff(hdeg.astype(np.int64)) >= 0)
+    # ties keep the id order
+    old = np.empty(n, np.int64)
+    old[new_id] = np.arange(n)
+    same = hdeg[1:] == hdeg[:-1]
+    assert np.all(old[1:][same] > old[:-1][same])
+    # every list is the image of the old list
+    for v in range(n):
+        assert np.array_equal(hc[hr[new_id[v]]:hr[new_id[v] + 1]], np.sort(new_id[cl[rp[v]:rp[v + 1]]]))
+    # and threads do not change the result
+    h2, id2 = g.relabel_by_degree(threads=1)
+    assert np.array_equal(id2, new_id) and all(np.array_equal(a, b) for a, b in zip(h2.csr(), (hr, hc)))
+
+
+def test_relabel_by_degree_keeps_counts_on_cpu(vsgm, oracle):
+    # the counts of every preset are a property of the graph, not of its ids
+    g = vsgm.Graph.rmat(8, 16, seed=8)
+    h, _ = g.relabel_by_degree()
+    rp, cl = g.csr()
+    hr, hc = h.csr()
+    for k in range(17):
+        text = vsgm.Plan.optimize(vsgm.Query.preset(k)).serialize()
+        assert oracle.count_plan(rp, cl, text) == oracle.count_plan(hr, hc, text), k
+
+
+def test_relabel_by_degree_shrinks_four_cycle_paths(vsgm, oracle):
+    # the 4-cycle plan roots each embedding at its smallest id; with hubs last
+    # the partial embeddings of its third level (two neighbours above the
+    # root) are far fewer on a skewed graph
+    g = vsgm.Graph.rmat(10, 16, seed=9)
+    h, _ = g.relabel_by_degree()
+    text = vsgm.Plan.optimize(vsgm.Query.preset(1)).serialize()
+    a = oracle.count_plan(*g.csr(), text, level_paths=True)
+    b = oracle.count_plan(*h.csr(), text, level_paths=True)
+    assert a[0] == b[0]
+    assert sum(b[1]) * 3 < sum(a[1])

@@ -86,6 +86,7 @@ def _declare(L):
         "vsgm_graph_from_edges": ([u64, P, u64, PP], C.c_int),
         "vsgm_graph_read_snap": ([C.c_char_p, C.c_int, C.c_int, C.c_int, PP], C.c_int),
         "vsgm_graph_drop_isolated": ([P, PP], C.c_int),
+        "vsgm_graph_relabel_degree": ([P, C.c_int, P, PP], C.c_int),
         "vsgm_graph_load": ([C.c_char_p, C.c_int, C.c_int, C.c_int, PP], C.c_int),
         "vsgm_graph_rmat": ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, u64, C.c_int, PP], C.c_int),
         "vsgm_graph_chung_lu": ([u64, u64, C.c_double, u64, C.c_int, PP], C.c_int),
@@ -245,6 +246,14 @@ class Graph(_Handle):
         h = C.c_void_p()
         _check(lib().vsgm_graph_drop_isolated(self._h, C.byref(h)), "drop_isolated")
         return Graph(h.value)
+
+    def relabel_by_degree(self, threads: int = 0):
+        """(isomorphic copy with ids in ascending (degree, id) order, new_id u32[V]): vertex v is new_id[v] in the
+        copy; pattern counts are unchanged."""
+        ids = np.empty(self.vertex_count, np.uint32)
+        h = C.c_void_p()
+        _check(lib().vsgm_graph_relabel_degree(self._h, threads, _ptr(ids), C.byref(h)), "relabel_degree")
+        return Graph(h.value), ids
 
     @classmethod
     def rmat(cls, scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1, threads: int = 0) -> "Graph":

@@ -114,6 +114,15 @@ int vsgm_graph_drop_isolated(const vsgm_graph* g, vsgm_graph** out) {
   });
 }
 
+int vsgm_graph_relabel_degree(const vsgm_graph* g, int threads, uint32_t* new_id, vsgm_graph** out) {
+  return Guard([&] {
+    VSGM_CHECK(g && out, "null argument");
+    std::vector<vsgm::uintV> id;
+    *out = new vsgm_graph{vsgm::RelabelByDegree(g->g, new_id ? &id : nullptr, threads)};
+    if (new_id) std::copy(id.begin(), id.end(), new_id);
+  });
+}
+
 int vsgm_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads, vsgm_graph** out) {
   return Guard([&] {
     VSGM_CHECK(out, "null argument");

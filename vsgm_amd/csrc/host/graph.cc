@@ -152,6 +152,37 @@ Graph BuildFromEdges(uint64_t vertex_count, std::vector<std::pair<uintV, uintV>>
   return g;
 }
 
+Graph RelabelByDegree(const Graph& g, std::vector<uintV>* new_id_out, int threads) {
+  const int nt = ThreadCount(threads);
+  const uint64_t n = g.vertex_count;
+  // counting sort by degree; ids ascending within one degree (stable)
+  const uint64_t maxd = g.MaxDegree();
+  std::vector<uint64_t> first(maxd + 2, 0);
+  for (uint64_t v = 0; v < n; ++v) ++first[g.Degree((uintV)v) + 1];
+  for (uint64_t d = 0; d <= maxd; ++d) first[d + 1] += first[d];
+  std::vector<uintV> order(n), new_id(n);
+  for (uint64_t v = 0; v < n; ++v) {
+    const uint64_t r = first[g.Degree((uintV)v)]++;
+    order[r] = (uintV)v;
+    new_id[v] = (uintV)r;
+  }
+  Graph out;
+  out.vertex_count = n;
+  out.row_ptrs.assign(n + 1, 0);
+  for (uint64_t r = 0; r < n; ++r) out.row_ptrs[r + 1] = out.row_ptrs[r] + g.Degree(order[r]);
+  out.cols.resize(g.cols.size());
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1024)
+  for (uint64_t r = 0; r < n; ++r) {
+    const uintV v = order[r];
+    uintV* dst = out.cols.data() + out.row_ptrs[r];
+    const uint64_t b = g.row_ptrs[v], e = g.row_ptrs[v + 1];
+    for (uint64_t i = b; i < e; ++i) dst[i - b] = new_id[g.cols[i]];
+    std::sort(dst, dst + (e - b));
+  }
+  if (new_id_out) *new_id_out = std::move(new_id);
+  return out;
+}
+
 Graph GenerateRMAT(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int threads) {
   VSGM_CHECK(scale >= 1 && scale <= 31, "R-MAT scale must be in [1, 31]");
   VSGM_CHECK(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0, "bad R-MAT probabilities");

@@ -36,6 +36,14 @@ void WriteCSRBinFile(const std::string& filename, const Graph& g);
 // isolated-vertex renumbering). threads <= 0 uses all cores.
 Graph BuildFromEdges(uint64_t vertex_count, std::vector<std::pair<uintV, uintV>>& edges, int threads = 0);
 
+// Isomorphic copy of g with vertex ids in ascending (degree, id) order:
+// new_id[v] is v's id in the copy, lists stay sorted. Counts of every
+// pattern are unchanged (symmetry conditions keep one embedding per instance
+// under any total order of the ids); with hubs last, a plan whose root is
+// the smallest id of its embedding starts from low-degree vertices and
+// enumerates far fewer partial embeddings. No reference counterpart.
+Graph RelabelByDegree(const Graph& g, std::vector<uintV>* new_id = nullptr, int threads = 0);
+
 // SplitMix64 finaliser; the generators draw every random number as
 // Mix(seed, stream, index), so results do not depend on thread count.
 uint64_t Mix64(uint64_t x);

@@ -85,6 +85,11 @@ int Run(int argc, char** argv) {
   // the bins listed there (when the bin's fingerprint, a hash of its
   // sources, still matches) and adds their counts.
   const std::string ck_file = fl.str("-ck", "");
+  // -rl 1 renumbers the vertices in ascending degree order after loading
+  // (same counts; plans rooted at the smallest id of an embedding then start
+  // from low-degree vertices). A .vbmap file still lists bins by the ids of
+  // the file as read.
+  const int relabel = fl.i("-rl", 0);
 
   int gpus = 0;
   vsgm_device_count(&gpus);
@@ -120,6 +125,12 @@ int Run(int argc, char** argv) {
   Graph graph = LoadGraph(filename, -1, thread_num, false);
   std::cout << "read graph " << filename << " V=" << graph.vertex_count << " entries=" << graph.cols.size() << " in " << Ms(t0) << " ms"
             << std::endl;
+  std::vector<uintV> new_id;
+  if (relabel) {
+    t0 = std::chrono::steady_clock::now();
+    graph = RelabelByDegree(graph, &new_id, thread_num);
+    std::cout << "ids relabelled in ascending degree order in " << Ms(t0) << " ms" << std::endl;
+  }
   ViewBinManager vbm(&graph, hop, thread_num);
   if (gpu_k > 0) {
     t0 = std::chrono::steady_clock::now();
@@ -135,7 +146,7 @@ int Run(int argc, char** argv) {
       vbm.SetPartition(gpu_k, assignment);
     }
   } else {
-    vbm.LoadViewBinPartition(ViewBinPartitionName(filename, hop));
+    vbm.LoadViewBinPartition(ViewBinPartitionName(filename, hop), relabel ? &new_id : nullptr);
   }
   if (do_reorder) vbm.Reorder();
   if (do_split) vbm.Split(consumers, split_times);

@@ -116,7 +116,7 @@ void ViewBinManager::SetPartition(int n_bins, const std::vector<int>& assignment
   }
 }
 
-void ViewBinManager::LoadViewBinPartition(const std::string& filename) {
+void ViewBinManager::LoadViewBinPartition(const std::string& filename, const std::vector<uintV>* new_id) {
   std::ifstream in(filename);
   VSGM_CHECK(in.good(), "cannot open view-bin partition " + filename);
   int n_bins = 0;
@@ -126,6 +126,12 @@ void ViewBinManager::LoadViewBinPartition(const std::string& filename) {
   for (auto& a : assignment) {
     in >> a;
     VSGM_CHECK(!in.fail(), "short partition file " + filename);
+  }
+  if (new_id) {
+    VSGM_CHECK(new_id->size() == assignment.size(), "id map does not match the graph");
+    std::vector<int> mapped(assignment.size());
+    for (size_t v = 0; v < assignment.size(); ++v) mapped[(*new_id)[v]] = assignment[v];
+    assignment.swap(mapped);
   }
   SetPartition(n_bins, assignment);
 }

@@ -75,7 +75,9 @@ class ViewBinManager {
  public:
   ViewBinManager(const Graph* g, int hop, int threads) : graph_(g), hop_(hop), threads_(threads) {}
   // Text file: bin count, then one bin id per vertex (view_bin_manager.h:30-67).
-  void LoadViewBinPartition(const std::string& filename);
+  // With new_id, line v of the file is the bin of vertex new_id[v] (a
+  // partition written for the graph before RelabelByDegree).
+  void LoadViewBinPartition(const std::string& filename, const std::vector<uintV>* new_id = nullptr);
   void SetPartition(int n_bins, const std::vector<int>& assignment);
   void Split(int device_num, int times);  // view_bin_manager.h:69-134
   void Reorder();                         // view_bin_manager.h:136-142
