@@ -124,6 +124,12 @@ def test_preprocess_graph_cli(vsgm, tmp_path):
     grp, gcols = g.csr()
     assert np.array_equal(grp, rp) and np.array_equal(gcols, cols)
     assert subprocess.run([exe, "-f", str(path), "-o", "7"], capture_output=True, timeout=60).returncode != 0
+    # -rl 1: the same graph numbered in ascending (degree, id) order
+    out = subprocess.run([exe, "-f", str(path), "-d", "0", "-o", "1", "-rl", "1"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    h = vsgm.Graph.read_bin(str(path) + ".bin")
+    want, _ = vsgm.Graph.from_csr(rp, cols).relabel_by_degree()
+    assert all(np.array_equal(x, y) for x, y in zip(h.csr(), want.csr()))
 
 
 def test_host_read_snap_tiny_file_many_threads(vsgm, tmp_path):
