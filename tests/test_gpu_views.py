@@ -9,9 +9,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("hop,rd", [(1, 0), (2, 0), (2, 3), (3, 2)])
-def test_device_view_equals_host_view(vsgm, gpu, hop, rd):
-    g = vsgm.Graph.rmat(10, 16, seed=7)
+@pytest.mark.parametrize("hop,rd,hubs", [(1, 0, False), (2, 0, False), (2, 3, False), (3, 2, False), (2, 0, True), (3, 2, True)])
+def test_device_view_equals_host_view(vsgm, gpu, hop, rd, hubs):
+    # hubs: a Chung-Lu graph whose 15 largest lists (up to 41,017 entries)
+    # are long enough to be spread over the whole grid when expanded and copied
+    g = vsgm.Graph.chung_lu(1 << 16, 1 << 21, 2.1, seed=5) if hubs else vsgm.Graph.rmat(10, 16, seed=7)
     V = g.vertex_count
     k = 4
     assign = np.random.default_rng(hop * 10 + rd).integers(0, k, size=V).astype(np.int32)

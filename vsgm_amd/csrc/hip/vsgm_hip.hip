@@ -560,17 +560,66 @@ __global__ __launch_bounds__(kBlock) void k_view_seed(const uint64_t* __restrict
   }
 }
 
+// Lists of at least kViewBig entries are not walked by one wavefront (a hub
+// of millions of entries would leave one wave running long after the rest):
+// the per-vertex pass queues them (up to kViewQueue per pass; more are
+// walked in place) and k_view_big spreads each queued list over the grid.
+constexpr uint64_t kViewBig = 8192;
+constexpr uint32_t kViewQueue = 16384;
+constexpr unsigned kViewBigGrid = 1024;
+
+// Queues v (wave-uniform) when its list is long and the queue has room.
+__device__ inline bool view_queue_big(uint32_t v, uint64_t n, uint32_t* queue, unsigned int* qn, int lane) {
+  if (n < kViewBig) return false;
+  uint32_t slot = 0;
+  if (lane == 0) slot = atomicAdd(qn, 1u);
+  slot = __shfl(slot, 0);
+  if (slot >= kViewQueue) return false;
+  if (lane == 0) queue[slot] = v;
+  return true;
+}
+
 // One wavefront per vertex of the previous level; lanes stride its list.
 // Several lanes may mark the same vertex: they all store the same level.
 __global__ __launch_bounds__(kBlock) void k_view_expand(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ cols,
-                                                        uint64_t V, uint8_t level, uint8_t* __restrict__ dist) {
+                                                        uint64_t V, uint8_t level, uint8_t* __restrict__ dist, uint32_t* __restrict__ queue,
+                                                        unsigned int* __restrict__ qn) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
   for (uint64_t v = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < V; v += waves) {
     if (dist[v] != (uint8_t)(level - 1)) continue;
-    for (uint64_t j = row_ptrs[v] + lane; j < row_ptrs[v + 1]; j += kWave) {
+    const uint64_t b = row_ptrs[v], e = row_ptrs[v + 1];
+    if (view_queue_big((uint32_t)v, e - b, queue, qn, lane)) continue;
+    for (uint64_t j = b + lane; j < e; j += kWave) {
       const uint32_t w = cols[j];
       if (dist[w] == kFar) dist[w] = level;
+    }
+  }
+}
+
+// The queued lists, each spread over every wavefront of the grid (COPY:
+// copied into the view at vrp[v]; else: their far vertices marked level).
+// The starting wave is rotated per list so short queued lists land on
+// different waves.
+template <bool COPY>
+__global__ __launch_bounds__(kBlock) void k_view_big(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ cols,
+                                                     const uint32_t* __restrict__ queue, const unsigned int* __restrict__ qn, uint8_t level,
+                                                     uint8_t* __restrict__ dist, const uint64_t* __restrict__ vrp, uint32_t* __restrict__ vcols) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
+  const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const uint32_t n = min(*qn, kViewQueue);
+  for (uint32_t q = 0; q < n; ++q) {
+    const uint32_t v = queue[q];
+    const uint64_t b = row_ptrs[v], len = row_ptrs[v + 1] - b;
+    const uint64_t w0 = (wave + (uint64_t)q * 37) % waves;
+    for (uint64_t j = w0 * kWave + lane; j < len; j += waves * kWave) {
+      if (COPY) {
+        vcols[vrp[v] + j] = cols[b + j];
+      } else {
+        const uint32_t w = cols[b + j];
+        if (dist[w] == kFar) dist[w] = level;
+      }
     }
   }
 }
@@ -589,15 +638,17 @@ __global__ __launch_bounds__(kBlock) void k_view_len(const uint64_t* __restrict_
   if (threadIdx.x == 0 && s) atomicAdd(n_in, (unsigned long long)s);
 }
 
-// One wavefront per vertex inside the view copies its list (coalesced).
+// One wavefront per vertex inside the view copies its list (coalesced);
+// long lists are queued for k_view_big.
 __global__ __launch_bounds__(kBlock) void k_view_copy(const uint64_t* __restrict__ row_ptrs, const uint32_t* __restrict__ cols,
                                                       const uint8_t* __restrict__ dist, uint64_t V, const uint64_t* __restrict__ vrp,
-                                                      uint32_t* __restrict__ vcols) {
+                                                      uint32_t* __restrict__ vcols, uint32_t* __restrict__ queue, unsigned int* __restrict__ qn) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / kWave);
   for (uint64_t v = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; v < V; v += waves) {
     if (dist[v] == kFar) continue;
     const uint64_t b = row_ptrs[v], n = row_ptrs[v + 1] - b, o = vrp[v];
+    if (view_queue_big((uint32_t)v, n, queue, qn, lane)) continue;
     for (uint64_t j = lane; j < n; j += kWave) vcols[o + j] = cols[b + j];
   }
 }
@@ -3420,7 +3471,12 @@ int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d
   Temps t(ctx);
   uint8_t* dist;
   uint64_t* vrp = nullptr;
+  uint32_t* queue;
+  uint64_t* qword;  // the queue's counter, a u64 word cleared by k_fill
   VSGM_TRY(t.get((V + 7) / 8 * 8, &dist));
+  VSGM_TRY(t.get(kViewQueue, &queue));
+  VSGM_TRY(t.get(1, &qword));
+  unsigned int* qn = (unsigned int*)qword;
   VSGM_TRY(pool_alloc(ctx, (V + 1) * sizeof(uint64_t), (void**)&vrp));
   auto undo = [&](int code) {
     pool_free(ctx, vrp);
@@ -3438,9 +3494,17 @@ int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d
     if ((s = launch_check(ctx, "k_view_seed")) != VSGM_OK) return undo(s);
   }
   const unsigned wave_grid = grid_for((V + kBlock / kWave - 1) / (kBlock / kWave));
+  auto clear_queue = [&]() {
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, ctx->stream, qword, 1, 0);
+    return launch_check(ctx, "k_fill");
+  };
   for (int level = 1; level < hop; ++level) {
-    hipLaunchKernelGGL(k_view_expand, dim3(wave_grid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, V, (uint8_t)level, dist);
+    if ((s = clear_queue()) != VSGM_OK) return undo(s);
+    hipLaunchKernelGGL(k_view_expand, dim3(wave_grid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, V, (uint8_t)level, dist, queue, qn);
     if ((s = launch_check(ctx, "k_view_expand")) != VSGM_OK) return undo(s);
+    hipLaunchKernelGGL(k_view_big<false>, dim3(kViewBigGrid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, queue, qn, (uint8_t)level,
+                       dist, (const uint64_t*)nullptr, (uint32_t*)nullptr);
+    if ((s = launch_check(ctx, "k_view_big")) != VSGM_OK) return undo(s);
   }
   hipLaunchKernelGGL(k_view_len, dim3(grid_for((V + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, d_row_ptrs, dist, V, vrp,
                      (unsigned long long*)(ctx->d_stats + 1));
@@ -3452,8 +3516,17 @@ int vsgm_view_build(vsgm_ctx* ctx, const uint64_t* d_row_ptrs, const uint32_t* d
   if ((s = host_read(ctx, ctx->d_stats, vals, 2)) != VSGM_OK) return undo(s);
   uint32_t* vcols = nullptr;
   if ((s = pool_alloc(ctx, std::max<uint64_t>(vals[0], 1) * sizeof(uint32_t), (void**)&vcols)) != VSGM_OK) return undo(s);
-  hipLaunchKernelGGL(k_view_copy, dim3(wave_grid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, dist, V, vrp, vcols);
-  if ((s = launch_check(ctx, "k_view_copy")) != VSGM_OK) {
+  if ((s = clear_queue()) != VSGM_OK) {
+    pool_free(ctx, vcols);
+    return undo(s);
+  }
+  hipLaunchKernelGGL(k_view_copy, dim3(wave_grid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, dist, V, vrp, vcols, queue, qn);
+  if ((s = launch_check(ctx, "k_view_copy")) == VSGM_OK) {
+    hipLaunchKernelGGL(k_view_big<true>, dim3(kViewBigGrid), dim3(kBlock), 0, ctx->stream, d_row_ptrs, d_cols, queue, qn, (uint8_t)0, dist,
+                       vrp, vcols);
+    s = launch_check(ctx, "k_view_big");
+  }
+  if (s != VSGM_OK) {
     pool_free(ctx, vcols);
     return undo(s);
   }
