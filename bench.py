@@ -654,10 +654,11 @@ def chunglu_main(args):
                                                 "match_s": round(m3, 3), "view_s": round(b3, 3)}
     # 4-clique and house over growing samples of every 2-hop bin's sources.
     # The house sample is a fraction of each bin rounded down, so bins of a
-    # few hubs (whose house counts run past 2^64) join only at large fractions.
+    # few hubs (a hub root alone enumerates more houses than a pass can
+    # count in minutes) join only at large fractions.
     rng = np.random.default_rng(args.seed)
     order = [rng.permutation(np.nonzero(bin_of == b)[0].astype(np.uint32)) for b in range(nb)]
-    for k, first in (("kc4", lambda o: max(1, len(o) // 4096)), ("p4", lambda o: len(o) >> 18)):
+    for k, first in (("kc4", lambda o: max(1, len(o) // 4096)), ("p4", lambda o: len(o) >> 20)):
         take = [first(o) for o in order]
         if sum(take) == 0:
             take[int(np.argmax([len(o) for o in order]))] = 1
