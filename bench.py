@@ -243,7 +243,7 @@ def main():
                     help="with --relabel degree: first run one step over the graph with its ids as generated and "
                          "report it beside the timed steps")
     ap.add_argument("--n-patterns", type=int, default=64)
-    ap.add_argument("--share-prefixes", type=int, default=1, help="batched: match the plans as a prefix tree")
+    ap.add_argument("--share-prefixes", type=int, default=0, help="batched: match the plans as a prefix tree")
     ap.add_argument("--bins", type=int, default=8)
     args = ap.parse_args()
     os.environ["VSGM_PROGRESS"] = str(args.progress)
