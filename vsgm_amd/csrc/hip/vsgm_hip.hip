@@ -2062,32 +2062,54 @@ constexpr uint32_t kNoCand = 0xFFFFFFFEu;
 template <int R>
 __device__ inline void flat_rows_test(const vsgm_level_args& a, const FlatTable& h, uint32_t mode, uint64_t t, uint32_t n, uint32_t* c) {
   if (mode == kModeHeld) {
+    // both tables' reads of every row are issued before any is compared, so
+    // the rows' LDS latencies overlap (a short-circuit per row would wait
+    // for each row's first read before issuing its second and the next row)
+    uint32_t s0[R], s1[R];
 #pragma unroll
-    for (int it = 0; it < R; ++it) c[it] = flat_contains(h, c[it]) ? c[it] : kNoCand;
+    for (int it = 0; it < R; ++it) s0[it] = h.t0[flat_h0(c[it], h.sh)];
+#pragma unroll
+    for (int it = 0; it < R; ++it) s1[it] = h.t1[flat_h1(c[it], h.sh)];
+#pragma unroll
+    for (int it = 0; it < R; ++it) c[it] = ((s0[it] == c[it]) | (s1[it] == c[it])) ? c[it] : kNoCand;
   } else if (mode == kModeIndex) {
+    // every row's word is loaded (a row without a candidate reads word 0 and
+    // stays kNoCand), so the loads issue back to back without branches
     const uint32_t* row = a.hub_bits + t;
     uint32_t w[R];
 #pragma unroll
-    for (int it = 0; it < R; ++it) w[it] = c[it] != kNoCand ? row[c[it] >> 5] : 0u;
+    for (int it = 0; it < R; ++it) w[it] = row[(c[it] != kNoCand ? c[it] : 0u) >> 5];
 #pragma unroll
     for (int it = 0; it < R; ++it) c[it] = ((w[it] >> (c[it] & 31)) & 1u) ? c[it] : kNoCand;
   } else if (mode == kModeSearch) {
-    // one list for all rows: binary lifting, every row's probe of a step in flight together
+    // one list for all rows: binary lifting, every row's probe of a step in
+    // flight together. Probe indices are clamped into the list, so each
+    // step's loads are unconditional and issue back to back; a probe past
+    // the end is loaded but not taken. n is the same for every lane.
     const uint32_t* L = a.cols + t;
+    if (n == 0) {
+#pragma unroll
+      for (int it = 0; it < R; ++it) c[it] = kNoCand;
+      return;
+    }
     uint32_t pos[R];
 #pragma unroll
     for (int it = 0; it < R; ++it) pos[it] = 0;
     for (uint32_t step = top_step(n); step; step >>= 1) {
+      uint32_t v[R];
+#pragma unroll
+      for (int it = 0; it < R; ++it) v[it] = L[min(pos[it] + step, n) - 1];
 #pragma unroll
       for (int it = 0; it < R; ++it) {
         const uint32_t probe = pos[it] + step;
-        const bool in = probe <= n;
-        const uint32_t v = L[in ? probe - 1 : 0];
-        pos[it] = (in && v < c[it]) ? probe : pos[it];
+        pos[it] = (probe <= n && v[it] < c[it]) ? probe : pos[it];
       }
     }
+    uint32_t f[R];
 #pragma unroll
-    for (int it = 0; it < R; ++it) c[it] = (pos[it] < n && L[pos[it]] == c[it]) ? c[it] : kNoCand;
+    for (int it = 0; it < R; ++it) f[it] = L[min(pos[it], n - 1)];
+#pragma unroll
+    for (int it = 0; it < R; ++it) c[it] = (pos[it] < n && f[it] == c[it]) ? c[it] : kNoCand;
   }
 }
 
@@ -2155,10 +2177,17 @@ __device__ inline bool flat_block(const vsgm_level_args& a, const FlatTable& h, 
   if (__ballot(span && !same) == 0) {
     if (ci == cj) {
       const uint32_t* src = a.cols + readlane_u64(w.base, ci) + x;
+      if (lim == (uint32_t)kFlatBlock) {  // a full block (uniform): no per-row bound
 #pragma unroll
-      for (int it = 0; it < kFlatRows; ++it) {
-        const uint32_t r = (uint32_t)it * kWave + lane;
-        c[it] = r < lim ? src[r] : kNoCand;
+        for (int it = 0; it < kFlatRows; ++it) c[it] = src[(uint32_t)it * kWave + lane];
+      } else {
+        // rows past the end load the block's last element and are dropped
+#pragma unroll
+        for (int it = 0; it < kFlatRows; ++it) {
+          const uint32_t r = (uint32_t)it * kWave + lane;
+          const uint32_t v = src[min(r, lim - 1)];
+          c[it] = r < lim ? v : kNoCand;
+        }
       }
     } else {
       // element r of the block lies in path ci + #{path starts in (x, x + r]}
@@ -2170,11 +2199,19 @@ __device__ inline bool flat_block(const vsgm_level_args& a, const FlatTable& h, 
 #pragma unroll
         for (int it = 0; it < kFlatRows; ++it) pi[it] += ((uint32_t)it * kWave + lane >= s) ? 1u : 0u;
       }
+      // every row's segment base is gathered first (with every lane active),
+      // then every row's candidate is loaded; a row past the end loads the
+      // block's last element (it lies in path cj, which such a row's pi is)
+      // and is dropped
+      uint64_t bse[kFlatRows];
+#pragma unroll
+      for (int it = 0; it < kFlatRows; ++it) bse[it] = shfl_u64(w.base, pi[it]);
+      const uint32_t* src = a.cols + x;
 #pragma unroll
       for (int it = 0; it < kFlatRows; ++it) {
         const uint32_t r = (uint32_t)it * kWave + lane;
-        const uint64_t bse = shfl_u64(w.base, pi[it]);  // gathered with every lane active
-        c[it] = r < lim ? a.cols[bse + x + r] : kNoCand;
+        const uint32_t v = src[bse[it] + min(r, lim - 1)];
+        c[it] = r < lim ? v : kNoCand;
       }
     }
     return true;

@@ -108,6 +108,7 @@ PipelineExecutor::~PipelineExecutor() {
   }
   prefetched_.reset();
   ClearViews();
+  vsgm_hub_index_free(ctx_, &graph_hub_);
   graph_rp_.reset();
   graph_cols_.reset();
   if (d_counts_) vsgm_free(ctx_, d_counts_);
@@ -167,7 +168,8 @@ void PipelineExecutor::ClearViews() {
   // background build must not be using meanwhile
   JoinPrefetch();
   ReleaseGraph();
-  for (auto& v : views_) vsgm_hub_index_free(ctx_, &v.hub);
+  for (auto& v : views_)
+    if (!v.hub_shared) vsgm_hub_index_free(ctx_, &v.hub);
   views_.clear();
   cur_ = 0;
   view_ = nullptr;
@@ -229,11 +231,56 @@ void PipelineExecutor::SetGraph(const Graph& g) {
   JoinPrefetch();  // a background build reads the resident graph
   prefetched_.reset();
   ReleaseGraph();
+  bool shared = false;  // device views of the previous graph share its index
+  for (const auto& v : views_) shared = shared || v.hub_shared;
+  if (shared) ClearViews();
+  vsgm_hub_index_free(ctx_, &graph_hub_);
+  graph_hub_built_ = false;
   graph_rp_ = std::make_shared<DeviceBuffer>(ctx_, (vertex_count_ + 1) * sizeof(uintE));
   graph_cols_ = std::make_shared<DeviceBuffer>(ctx_, std::max<uint64_t>(g.cols.size(), 1) * sizeof(uintV));
   Check(vsgm_h2d(ctx_, graph_rp_->get(), g.row_ptrs.data(), (vertex_count_ + 1) * sizeof(uintE)), "row_ptrs upload");
   Check(vsgm_h2d(ctx_, graph_cols_->get(), g.cols.data(), g.cols.size() * sizeof(uintV)), "cols upload");
   Check(vsgm_ctx_sync(ctx_), "graph upload");
+  uint64_t loops = 0;
+  Check(vsgm_view_self_loops(ctx_, (const uint64_t*)graph_rp_->get(), (const uint32_t*)graph_cols_->get(), vertex_count_, &loops),
+        "self-loop check");
+  graph_loop_free_ = loops == 0;
+  // index candidates in the order IndexView takes them (degree, then id),
+  // kept up to 2^38 bytes of index rows (at least 65,536 rows)
+  std::vector<std::pair<uint64_t, uintV>> cand;
+  for (uint64_t u = 0; u < vertex_count_; ++u) {
+    const uint64_t d = g.row_ptrs[u + 1] - g.row_ptrs[u];
+    if (d > 0) cand.emplace_back(d, (uintV)u);
+  }
+  const uint64_t row_bytes = (vertex_count_ + 63) / 64 * 8;
+  const size_t keep = (size_t)std::min<uint64_t>(cand.size(), std::max<uint64_t>(65536, (1ull << 38) / std::max<uint64_t>(row_bytes, 1)));
+  auto by_degree = [](const auto& x, const auto& y) { return x.first > y.first || (x.first == y.first && x.second < y.second); };
+  std::partial_sort(cand.begin(), cand.begin() + keep, cand.end(), by_degree);
+  cand.resize(keep);
+  graph_hubs_ = std::move(cand);
+}
+
+// A device view's membership index and loop flag are the resident graph's:
+// its lists are copies of the graph's, and the kernels only probe the index
+// rows of vertices whose lists the view holds.
+void PipelineExecutor::UseGraphIndex(View& v) {
+  if (!graph_hub_built_) {
+    const uint64_t row_bytes = (vertex_count_ + 63) / 64 * 8;
+    const uint64_t cap = std::min<uint64_t>(HubLimit(ctx_, opt_.hub_bytes, 1) / row_bytes, 0xFFFFFFFEull);
+    std::vector<uint32_t> hubs;
+    for (const auto& [d, u] : graph_hubs_) {
+      if (hubs.size() >= cap || d < std::max<uint64_t>(opt_.hub_min_degree, 1)) break;
+      hubs.push_back(u);
+    }
+    if (!hubs.empty())
+      Check(vsgm_hub_index_build(ctx_, (const uint64_t*)graph_rp_->get(), (const uint32_t*)graph_cols_->get(), vertex_count_, hubs.data(),
+                                 (uint32_t)hubs.size(), &graph_hub_),
+            "hub index build");
+    graph_hub_built_ = true;
+  }
+  v.hub = graph_hub_;
+  v.hub_shared = true;
+  v.loop_free = graph_loop_free_;
 }
 
 size_t PipelineExecutor::AddDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree) {
@@ -251,7 +298,7 @@ void PipelineExecutor::TransferDevice(const std::vector<uintV>& sources, int hop
   if (!hit) b.reset();  // a prefetch of another bin is dropped
   ClearViews();
   if (hit) {
-    IndexView(b->view, b->h_row_ptrs.data(), HubLimit(ctx_, opt_.hub_bytes, 1));
+    UseGraphIndex(b->view);
     views_.push_back(std::move(b->view));
   } else {
     views_.push_back(BuildDeviceView(sources, hop, root_degree, HubLimit(ctx_, opt_.hub_bytes, 1)));
@@ -312,8 +359,6 @@ PipelineExecutor::BuiltView PipelineExecutor::BuildViewOn(vsgm_ctx* ctx, const s
         "view build");
   v.row_ptrs = std::make_shared<DeviceBuffer>(ctx, (void*)o.row_ptrs);
   v.cols = std::make_shared<DeviceBuffer>(ctx, (void*)o.cols);
-  b.h_row_ptrs.resize(vertex_count_ + 1);
-  check(vsgm_d2h(ctx, b.h_row_ptrs.data(), o.row_ptrs, b.h_row_ptrs.size() * sizeof(uint64_t)), "view row_ptrs readback");
   check(vsgm_ctx_sync(ctx), "view build");
   return b;
 }
@@ -322,8 +367,9 @@ PipelineExecutor::View PipelineExecutor::BuildDeviceView(const std::vector<uintV
                                                          uint64_t hub_limit) {
   VSGM_CHECK(graph_rp_ && graph_cols_, "device view before SetGraph");
   CheckSources(sources.data(), sources.size());
+  (void)hub_limit;
   BuiltView b = BuildViewOn(ctx_, sources, hop, root_degree);
-  IndexView(b.view, b.h_row_ptrs.data(), hub_limit);
+  UseGraphIndex(b.view);
   return std::move(b.view);
 }
 

@@ -147,17 +147,18 @@ class PipelineExecutor {
     std::shared_ptr<DeviceBuffer> row_ptrs, cols, sources;
     uint64_t sources_num = 0;
     vsgm_hub_index hub{nullptr, nullptr, 0, 0};
-    bool loop_free = false;  // no v in adj(v): '!=' against backward vertices is implied
+    bool hub_shared = false;  // hub is the whole graph's index (not freed with the view)
+    bool loop_free = false;   // no v in adj(v): '!=' against backward vertices is implied
   };
   // A device view built on some context, before its membership index.
   struct BuiltView {
     View view;
-    std::vector<uint64_t> h_row_ptrs;
   };
   BuiltView BuildViewOn(vsgm_ctx* ctx, const std::vector<uintV>& sources, int hop, uint64_t root_degree) const;
   void JoinPrefetch();
   void ClearViews();
   void IndexView(View& v, const uint64_t* h_row_ptrs, uint64_t hub_limit);
+  void UseGraphIndex(View& v);
   View BuildDeviceView(const std::vector<uintV>& sources, int hop, uint64_t root_degree, uint64_t hub_limit);
   View UploadView(const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n_cols, const uint32_t* sources, uint64_t n_sources,
                   uint64_t hub_limit);
@@ -191,6 +192,15 @@ class PipelineExecutor {
   std::vector<View> views_;
   size_t cur_ = 0;
   std::shared_ptr<DeviceBuffer> graph_rp_, graph_cols_;  // whole graph (SetGraph)
+  // Device views copy whole lists of the resident graph, so one membership
+  // index of the graph's largest lists serves every one of them: built at
+  // the first device view after SetGraph, freed with the graph.
+  // the largest lists (degree, vertex), largest first: every index the
+  // options can ask for at the default size and more
+  std::vector<std::pair<uint64_t, uint32_t>> graph_hubs_;
+  vsgm_hub_index graph_hub_{nullptr, nullptr, 0, 0};
+  bool graph_hub_built_ = false;
+  bool graph_loop_free_ = false;
   const View* view_ = nullptr;  // view being matched
 
   const Plan* plan_ = nullptr;  // plan being matched
