@@ -662,14 +662,19 @@ def chunglu_main(args):
         take = [first(o) for o in order]
         if sum(take) == 0:
             take[int(np.argmax([len(o) for o in order]))] = 1
-        best = None
+        best, prev = None, None
         while True:
             sample = [np.sort(o[:n]) for o, n in zip(order, take)]
             n_src = int(sum(len(x) for x in sample))
             counts, wall, match, build, _ = run_bins([plans[k]], sample, plans[k].hop, f"{NAMES[k]}, {n_src} sampled sources")
             best = {"count": counts[0], "sample_sources": n_src, "sample_fraction": n_src / g.vertex_count,
                     "seconds": round(wall, 3), "match_s": round(match, 3), "embeddings_per_s": counts[0] / match if match else 0.0}
-            if n_src >= g.vertex_count or wall * 4 > args.cl_sample_s / 2 or elapsed() + wall * 4 > args.budget_s:
+            # the next pass (4x the sources) is predicted to grow like the
+            # last one did, at least 4x: counts grow faster than linearly
+            # once the sample reaches denser parts of the graph
+            grow = max(4.0, wall / prev) if prev else 4.0
+            prev = max(wall, 1e-3)
+            if n_src >= g.vertex_count or wall * grow > args.cl_sample_s / 2 or elapsed() + wall * grow > args.budget_s:
                 break
             take = [min(len(o), 4 * n) for o, n in zip(order, take)]
         out[NAMES[k]] = best
