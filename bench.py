@@ -676,7 +676,10 @@ def chunglu_main(args):
             prev = max(wall, 1e-3)
             if n_src >= g.vertex_count or wall * grow > args.cl_sample_s / 2 or elapsed() + wall * grow > args.budget_s:
                 break
-            take = [min(len(o), 4 * n) for o, n in zip(order, take)]
+            grown = [min(len(o), 4 * n) for o, n in zip(order, take)]
+            if grown == take:  # every bin the sample draws from is whole
+                break
+            take = grown
         out[NAMES[k]] = best
     # parity: all three plans against the oracle on seeded roots
     roots = np.sort(rng.permutation(g.vertex_count)[: args.cl_parity_roots]).astype(np.uint32)
