@@ -221,7 +221,9 @@ def main():
     ap.add_argument("--hub-bytes", type=float, default=-1, help="membership index bytes (-1 auto, 0 off)")
     ap.add_argument("--hub-min-degree", type=int, default=64)
     ap.add_argument("--isect-kernel", type=int, default=4)
-    ap.add_argument("--cold-weight", type=int, default=16)
+    ap.add_argument("--cold-weight", type=int, default=None,
+                    help="pivot weight of a list that changes from path to path (default: 1 with degree-order ids, "
+                         "16 with ids as generated, the fastest of 1/4/16/64 on each)")
     ap.add_argument("--ticket-chunks", type=int, default=64)
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines of long map launches")
     ap.add_argument("--cl-vertices", type=int, default=65_600_000)
@@ -248,6 +250,9 @@ def main():
     args = ap.parse_args()
     os.environ["VSGM_PROGRESS"] = str(args.progress)
 
+    cold_given = args.cold_weight is not None
+    if not cold_given:
+        args.cold_weight = 1 if args.relabel == "degree" else 16
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path shards by view bins "
               f"across GPUs with no data exchange; only the single-GPU share is built and measured here)", flush=True)
@@ -289,7 +294,7 @@ def main():
     hop = max(p.hop for p in plans)
     root_degree = min(p.root_degree for p in plans)
 
-    def single_view(graph):
+    def single_view(graph, cold_weight):
         """One k-means partition with k = 1 on the GPU (every vertex in
         cluster 0), packed as one view bin (the hop-hop view with the
         root-degree filter) and transferred to a new executor."""
@@ -306,7 +311,7 @@ def main():
         ex.set_option("hub_bytes", args.hub_bytes)
         ex.set_option("hub_min_degree", args.hub_min_degree)
         ex.set_option("isect_kernel", args.isect_kernel)
-        ex.set_option("cold_weight", args.cold_weight)
+        ex.set_option("cold_weight", cold_weight)
         ex.set_option("ticket_chunks", args.ticket_chunks)
         t = time.perf_counter()
         ex.transfer(holder)
@@ -320,17 +325,20 @@ def main():
     # the timed steps; the timed steps get the budget left after it.
     generated_ids = None
     if args.relabel != "none" and args.generated_ids_step:
-        gx, _, _ = single_view(g_generated)
+        gen_cold = args.cold_weight if cold_given else 16
+        gx, _, _ = single_view(g_generated, gen_cold)
         t = time.perf_counter()
         gx.match()
         dt = time.perf_counter() - t
         generated_ids = {"ms_per_step": round(dt * 1000.0, 1), "counts": gx.total_counts,
                          "per_pattern_ms": {k: round(m, 1) for k, m in zip(keys, gx.match_ms)},
-                         "note": "one untimed-budget step over the graph with its ids as generated, same plans and view"}
+                         "cold_weight": gen_cold,
+                         "note": "one step over the graph with its ids as generated, same plans and view, before the "
+                                 "timed steps"}
         log(f"ids as generated: one step {dt:.2f}s counts={gx.total_counts}")
         del gx
 
-    ex, view_s, xfer_s = single_view(g)
+    ex, view_s, xfer_s = single_view(g, args.cold_weight)
 
     # CPU baseline (and the parity sample) before the GPU steps, so the steps
     # get whatever budget is left
