@@ -279,6 +279,15 @@ int Run(int argc, char** argv) {
         ExecutorOptions eo;
         eo.share_prefixes = share_prefixes != 0;
         PipelineExecutor exec(c, graph.vertex_count, plan_ptrs, max_sources, eo);
+        if (relabel) {
+          // with degree-order ids every list is short: the shortest list is
+          // the cheapest pivot (weight 1 for lists that change per path)
+          for (size_t i = 0; i < plan_ptrs.size(); ++i) {
+            PlanTuning t = exec.GetPlanTuning(i);
+            t.cold_weight = 1;
+            exec.SetPlanTuning(i, t);
+          }
+        }
         if (device_views) {
           exec.SetGraph(graph);
           // the next bin's view is built in the background while this one matches
