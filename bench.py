@@ -538,7 +538,33 @@ def learned_plans(V, args, keys, queries, g):
     vstats = learned.view_stats(rp_all, g.vertex_count)
     planner = learned.LearnedPlanner(model, margin=margin)
     chosen = [planner.plan(q, vstats) for q in queries]
+    # a learned order is kept only if it also beats the heuristic order on a
+    # seeded sample of this graph's own roots (the records come from far
+    # smaller graphs, whose costs can rank orders differently)
+    roots = np.sort(np.random.default_rng(args.seed + 7).permutation(g.vertex_count)[: max(1024, g.vertex_count // 64)])
+    roots = roots.astype(np.uint32)
+    checks = {}
+    for i, (k, q, (p, c)) in enumerate(zip(keys, queries, chosen)):
+        if not c["learned"]:
+            continue
+        h = V.Plan.optimize(q)
+        ms, cnt = {}, {}
+        for name, plan in (("learned", p), ("heuristic", h)):
+            ex = V.PipelineExecutor(0, g.vertex_count, [plan])
+            ex.set_option("isect_kernel", args.isect_kernel)
+            ex.set_option("cold_weight", args.cold_weight)
+            ex.transfer_graph(g, roots)
+            ex.match()
+            ms[name], cnt[name] = ex.match_ms[0], ex.total_counts[0]
+            del ex
+        assert cnt["learned"] == cnt["heuristic"], f"{k}: learned and heuristic plans count differently on the sample"
+        keep = ms["learned"] < 0.95 * ms["heuristic"]
+        checks[k] = {"roots": int(len(roots)), "learned_ms": round(ms["learned"], 3), "heuristic_ms": round(ms["heuristic"], 3),
+                     "kept": keep}
+        if not keep:
+            chosen[i] = (h, {**c, "learned": False, "rejected_on_sample": True})
     info = {"records": int(len(data["y"])), "train_records": int(len(train_set["y"])), "margin": margin,
+            "sample_check": checks,
             "margin_table": table, "loss": [float(losses[0]), float(losses[-1])],
             "orders": {k: (c.get("order") if c["learned"] else "heuristic") for k, (_, c) in zip(keys, chosen)},
             "predicted_gain": {k: (round(c["pred_gain"], 4), c.get("best_order", c.get("order")))
