@@ -540,7 +540,10 @@ def learned_plans(V, args, keys, queries, g):
     chosen = [planner.plan(q, vstats) for q in queries]
     # a learned order is kept only if it also beats the heuristic order on a
     # seeded sample of this graph's own roots (the records come from far
-    # smaller graphs, whose costs can rank orders differently)
+    # smaller graphs, whose costs can rank orders differently). Each plan's
+    # time over uniformly drawn roots estimates its whole time; its count
+    # there is not comparable (the orders root an embedding at different
+    # pattern vertices), only the totals are.
     roots = np.sort(np.random.default_rng(args.seed + 7).permutation(g.vertex_count)[: max(1024, g.vertex_count // 64)])
     roots = roots.astype(np.uint32)
     checks = {}
@@ -548,16 +551,15 @@ def learned_plans(V, args, keys, queries, g):
         if not c["learned"]:
             continue
         h = V.Plan.optimize(q)
-        ms, cnt = {}, {}
+        ms = {}
         for name, plan in (("learned", p), ("heuristic", h)):
             ex = V.PipelineExecutor(0, g.vertex_count, [plan])
             ex.set_option("isect_kernel", args.isect_kernel)
             ex.set_option("cold_weight", args.cold_weight)
             ex.transfer_graph(g, roots)
             ex.match()
-            ms[name], cnt[name] = ex.match_ms[0], ex.total_counts[0]
+            ms[name] = ex.match_ms[0]
             del ex
-        assert cnt["learned"] == cnt["heuristic"], f"{k}: learned and heuristic plans count differently on the sample"
         keep = ms["learned"] < 0.95 * ms["heuristic"]
         checks[k] = {"roots": int(len(roots)), "learned_ms": round(ms["learned"], 3), "heuristic_ms": round(ms["heuristic"], 3),
                      "kept": keep}
