@@ -540,29 +540,16 @@ def learned_plans(V, args, keys, queries, g):
     chosen = [planner.plan(q, vstats) for q in queries]
     # a learned order is kept only if it also beats the heuristic order on a
     # seeded sample of this graph's own roots (the records come from far
-    # smaller graphs, whose costs can rank orders differently). Each plan's
-    # time over uniformly drawn roots estimates its whole time; its count
-    # there is not comparable (the orders root an embedding at different
-    # pattern vertices), only the totals are.
-    roots = np.sort(np.random.default_rng(args.seed + 7).permutation(g.vertex_count)[: max(1024, g.vertex_count // 64)])
-    roots = roots.astype(np.uint32)
+    # smaller graphs, whose costs can rank orders differently)
+    roots = learned.sample_roots(g.vertex_count, seed=args.seed + 7)
     checks = {}
     for i, (k, q, (p, c)) in enumerate(zip(keys, queries, chosen)):
         if not c["learned"]:
             continue
         h = V.Plan.optimize(q)
-        ms = {}
-        for name, plan in (("learned", p), ("heuristic", h)):
-            ex = V.PipelineExecutor(0, g.vertex_count, [plan])
-            ex.set_option("isect_kernel", args.isect_kernel)
-            ex.set_option("cold_weight", args.cold_weight)
-            ex.transfer_graph(g, roots)
-            ex.match()
-            ms[name] = ex.match_ms[0]
-            del ex
-        keep = ms["learned"] < 0.95 * ms["heuristic"]
-        checks[k] = {"roots": int(len(roots)), "learned_ms": round(ms["learned"], 3), "heuristic_ms": round(ms["heuristic"], 3),
-                     "kept": keep}
+        keep, ms_l, ms_h = learned.check_on_roots(g, p, h, roots, {"isect_kernel": args.isect_kernel,
+                                                                  "cold_weight": args.cold_weight})
+        checks[k] = {"roots": int(len(roots)), "learned_ms": round(ms_l, 3), "heuristic_ms": round(ms_h, 3), "kept": keep}
         if not keep:
             chosen[i] = (h, {**c, "learned": False, "rejected_on_sample": True})
     info = {"records": int(len(data["y"])), "train_records": int(len(train_set["y"])), "margin": margin,

@@ -78,3 +78,32 @@ def test_learned_planner(vsgm, oracle, gpu):
     assert c[:k] == c[k:]
     print("[gpu] learned plans match the heuristic counts; oracle check", flush=True)
     assert c[k:] == [oracle.count_plan(rp, cl, p.serialize()) for p in heur]
+
+
+def test_root_sample_check(vsgm, gpu):
+    """The root-sample check runs both plans over the same sampled roots and
+    keeps the learned plan only when it beats the heuristic one by the given
+    margin: with a margin no plan can meet (99 %) it refuses, with one every
+    plan meets (-1000 %) it keeps, and both plans count the same over the
+    whole graph. Prints the two times of a 4-clique order against the
+    heuristic order on a degree-ordered R-MAT graph."""
+    from vsgm_amd import learned as L
+
+    g, _ = vsgm.Graph.rmat(14, 16, seed=31).relabel_by_degree()
+    q = vsgm.Query.clique(4)
+    heur, other = vsgm.Plan.optimize(q), vsgm.Plan.from_order(q, [2, 3, 1, 0])
+    roots = L.sample_roots(g.vertex_count, seed=5, fraction=4)
+    assert len(roots) == g.vertex_count // 4 and np.all(np.diff(roots.astype(np.int64)) > 0)
+    print(f"[gpu] root-sample check, R-MAT s14 degree ids, 4-clique, {len(roots)} roots", flush=True)
+    keep, ms_o, ms_h = L.check_on_roots(g, other, heur, roots, min_gain=0.99)
+    assert not keep and ms_o > 0 and ms_h > 0
+    keep, _, _ = L.check_on_roots(g, other, heur, roots, min_gain=-10.0)
+    assert keep
+    print(f"[gpu] order (2,3,1,0) {ms_o:.3f} ms, heuristic (0,1,2,3) {ms_h:.3f} ms", flush=True)
+    totals = []
+    for plan in (heur, other):
+        ex = vsgm.PipelineExecutor(0, g.vertex_count, [plan])
+        ex.transfer_graph(g)
+        ex.match()
+        totals.append(ex.total_counts[0])
+    assert totals[0] == totals[1]

@@ -194,6 +194,34 @@ def calibrate_margin(model: CostModel, held: Dict[str, np.ndarray], margins: Seq
     return best["margin"], table
 
 
+def sample_roots(vertex_count: int, seed: int = 0, fraction: int = 64, minimum: int = 1024) -> np.ndarray:
+    """A seeded uniform sample of root vertices (1/fraction of them, at
+    least `minimum`), sorted."""
+    n = min(vertex_count, max(minimum, vertex_count // fraction))
+    return np.sort(np.random.default_rng(seed).permutation(vertex_count)[:n]).astype(np.uint32)
+
+
+def check_on_roots(graph: "V.Graph", learned_plan: "V.Plan", heuristic_plan: "V.Plan", roots: np.ndarray,
+                   options: Optional[Dict[str, int]] = None, min_gain: float = 0.05):
+    """Times both plans over the same roots of the workload graph (the whole
+    graph as the view, the roots as sources) and keeps the learned plan only
+    when it is at least `min_gain` faster there. Each plan's time over
+    uniformly drawn roots estimates its whole time; its count over them is
+    not comparable between orders (each order roots an embedding at a
+    different pattern vertex), only whole-graph totals are. Returns (keep,
+    learned ms, heuristic ms)."""
+    ms = []
+    for plan in (learned_plan, heuristic_plan):
+        ex = V.PipelineExecutor(0, graph.vertex_count, [plan])
+        for k, v in (options or {}).items():
+            ex.set_option(k, v)
+        ex.transfer_graph(graph, roots)
+        ex.match()
+        ms.append(ex.match_ms[0])
+        del ex
+    return ms[0] < (1.0 - min_gain) * ms[1], ms[0], ms[1]
+
+
 class LearnedPlanner:
     """Picks matching orders with a trained CostModel; falls back to the
     heuristic plan when no order is predicted to beat it by `margin`."""
