@@ -123,3 +123,15 @@ def test_matrix_core_mode_stays_close_to_the_fp32_activation_model():
     l1, g1 = M.loss_and_grad(p, x, t, mfma=True)
     assert abs(l1 - l0) <= 0.03 * l0
     assert np.max(np.abs(g1 - g0)) <= 0.05 * np.max(np.abs(g0)) and not np.array_equal(g0, g1)
+
+
+def test_sample_roots_is_seeded_sorted_and_sized():
+    from vsgm_amd import learned as L
+
+    a = L.sample_roots(1 << 16, seed=3)
+    assert a.dtype == np.uint32 and len(a) == (1 << 16) // 64
+    assert np.all(np.diff(a.astype(np.int64)) > 0) and a.max() < (1 << 16)
+    assert np.array_equal(a, L.sample_roots(1 << 16, seed=3))
+    assert not np.array_equal(a, L.sample_roots(1 << 16, seed=4))
+    assert len(L.sample_roots(500, seed=1)) == 500  # fewer vertices than the minimum: all of them
+    assert len(L.sample_roots(1 << 12, seed=1, fraction=8, minimum=16)) == 512
