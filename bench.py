@@ -230,7 +230,7 @@ def main():
     ap.add_argument("--cl-edges", type=float, default=1.81e9, help="target undirected edges of the Chung-Lu graph")
     ap.add_argument("--cl-sample-s", type=float, default=120.0, help="Chung-Lu: seconds a sampled 4-clique / house pass may grow to")
     ap.add_argument("--cl-parity-roots", type=int, default=8, help="Chung-Lu: roots whose counts are checked against the oracle")
-    ap.add_argument("--workload", choices=["single", "batched", "copy-calibration", "chunglu"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "copy-calibration", "chunglu", "triangles-cpu"], default="single",
                     help="single: the 4-clique/diamond/4-cycle line; batched: many random patterns over k view bins, "
                          "one-at-a-time vs batched+tuned vs learned+batched; copy-calibration: kernels of known HBM "
                          "traffic (a 2 GiB fill and a 2 GiB reduction) that scale the rocprof byte counters")
@@ -264,6 +264,8 @@ def main():
         return calibration_main()
     if args.workload == "chunglu":
         return chunglu_main(args)
+    if args.workload == "triangles-cpu":
+        return triangles_cpu_main(args)
 
     import numpy as np
 
@@ -729,6 +731,59 @@ def chunglu_main(args):
                      "frac": round(min(achieved / PEAK_HBM_GBS, 1.0), 4), "traffic": None, "kernel": dom["name"],
                      "launches": dom["launches"], "avg_launch_ms": round(dom["total_ms"] / dom["launches"], 3)},
         "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def triangles_cpu_main(args):
+    """Triangle counting (-kc 3, heuristic plan) on R-MAT scale 16 (or
+    --scale), one view bin: the GPU path's count and rate beside a full run
+    of the oracle's C restatement of the level algorithm on min(32, cores)
+    host threads, both checked against the textbook triangle counter."""
+    import numpy as np
+
+    import oracle as O
+    import vsgm_amd as V
+    from vsgm_amd import partition
+
+    scale = args.scale if args.scale != 22 else 16
+    g = V.Graph.rmat(scale, args.edge_factor, 0.57, 0.19, 0.19, seed=args.seed)
+    plan = V.Plan.optimize(V.Query.clique(3))
+    assign, _, _ = partition.kmeans(g, 1, iters=1, seed=args.seed)
+    vbm = V.ViewBinManager.from_assignment(g, plan.hop, 1, assign)
+    holder = V.ViewBinHolder(g.vertex_count, max(4, g.entries * 4))
+    vbm.materialize(0, holder, plan.root_degree)
+    ex = V.PipelineExecutor(0, g.vertex_count, [plan], max_sources=vbm.max_sources)
+    ex.transfer(holder)
+    ex.match()  # warm-up
+    ex.reset()
+    steps = max(1, min(args.steps, 20))
+    t = time.perf_counter()
+    for _ in range(steps):
+        ex.match()
+    gpu_s = (time.perf_counter() - t) / steps
+    gpu_count = ex.total_counts[0] // steps
+    rp, cl = g.csr(copy=False)
+    # min(32, cores), within the host threads this process is given
+    threads = min(32, os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS") or 32))
+    t = time.perf_counter()
+    cpu_count = O.count_plan(rp, cl, plan.serialize(), threads=threads)
+    cpu_s = time.perf_counter() - t
+    textbook = int(O.triangles(rp, cl))
+    log(f"triangles R-MAT s{scale}: gpu {gpu_count} in {gpu_s * 1000:.2f} ms/step, cpu {cpu_count} in {cpu_s:.2f}s "
+        f"({threads} threads), textbook {textbook}")
+    assert gpu_count == cpu_count == textbook, (gpu_count, cpu_count, textbook)
+    line = {
+        "metric": "embeddings/sec per pattern, 1 MI355X; learned vs heuristic plan, exact counts",
+        "value": gpu_count / gpu_s, "unit": "embeddings/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": gpu_s * 1000.0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": f"synthetic: seeded R-MAT (seed {args.seed}) generated on the host",
+        "config": {"workload": f"triangles (-kc 3), heuristic plan, R-MAT scale {scale} ef {args.edge_factor} "
+                               f"(.57/.19/.19/.05), one view bin; the CPU path timed beside it",
+                   "count": gpu_count, "textbook_count": textbook, "vertices": g.vertex_count, "edges": g.edge_count},
+        "roofline": None,
+        "cpu_baseline": {"value": cpu_count / cpu_s, "unit": "embeddings/s", "cores": threads, "kind": "port",
+                         "sample": f"full run: every root, {cpu_count} triangles in {cpu_s:.2f} s"},
     }
     print(json.dumps(line), flush=True)
 
