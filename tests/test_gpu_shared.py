@@ -92,9 +92,9 @@ def test_pm_pipeline_graphlets_shared(vsgm, oracle, gpu, tmp_path):
 
 
 def test_shared_time_on_five_vertex_graphlets(vsgm, gpu):
-    """Printed: the 21 five-vertex graphlet plans on R-MAT s13, one at a time
+    """Printed: the 21 five-vertex graphlet plans on R-MAT s12, one at a time
     and as a prefix tree (same counts)."""
-    g = vsgm.Graph.rmat(13, 16, seed=26)
+    g = vsgm.Graph.rmat(12, 16, seed=26)
     plans = [vsgm.Plan.optimize(q) for q in vsgm.Query.graphlets(5)]
     t = time.perf_counter()
     plain, _ = _run(vsgm, g, plans, False)
@@ -102,6 +102,6 @@ def test_shared_time_on_five_vertex_graphlets(vsgm, gpu):
     t = time.perf_counter()
     shared, st = _run(vsgm, g, plans, True)
     t_shared = time.perf_counter() - t
-    print(f"[gpu] 21 five-vertex graphlets, R-MAT s13: one at a time {t_plain:.2f}s, prefix tree {t_shared:.2f}s "
+    print(f"[gpu] 21 five-vertex graphlets, R-MAT s12: one at a time {t_plain:.2f}s, prefix tree {t_shared:.2f}s "
           f"({st['saved_runs']} level runs saved)", flush=True)
     assert shared == plain
