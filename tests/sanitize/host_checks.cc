@@ -69,6 +69,18 @@ void GraphsAndIngest(const std::string& dir) {
   WriteCSRBinFile(dir + "/g.bin", g);
   Expect(SameGraph(ReadCSRBinFile(dir + "/g.bin"), g), ".bin round trip");
   Expect(GenerateChungLu(3000, 9000, 2.3, 2, 4).Validate().empty(), "Chung-Lu graph valid");
+  {
+    std::vector<uintV> id;
+    Graph r = RelabelByDegree(g, &id, 3);
+    Expect(r.Validate().empty() && r.cols.size() == g.cols.size() && id.size() == g.vertex_count, "relabelled graph valid");
+    bool ascending = true;
+    for (uint64_t v = 0; v + 1 < r.vertex_count; ++v) ascending = ascending && r.Degree((uintV)v) <= r.Degree((uintV)(v + 1));
+    Expect(ascending, "relabelled degrees ascend");
+    Expect(SameGraph(RelabelByDegree(g, nullptr, 1), r), "relabel independent of threads");
+    Graph empty;
+    empty.row_ptrs.assign(1, 0);
+    Expect(RelabelByDegree(empty).vertex_count == 0, "relabel of an empty graph");
+  }
 
   // SNAP text: comments, CRLF, trailing fields, a self loop, duplicates
   std::mt19937_64 rng(5);
