@@ -180,3 +180,30 @@ def test_level_paths_are_reported(vsgm, oracle):
     assert total == 10
     assert paths[0] == 5  # one partial embedding per root
     assert paths[3] == 10  # (v0 < v1) edge paths reaching the last level
+
+
+def test_plan_restatement_equals_golden_counts_at_scale_16(vsgm, oracle):
+    """The level-by-level restatement (heuristic plans) against the golden
+    textbook counts of the R-MAT scale-16 graph (tests/golden/rmat_counts.json,
+    graph identified by its CSR digest), on the graph as generated for the
+    triangles and on its degree-order relabelling for the triangle, 4-clique
+    and diamond (the 4-cycle's plan takes minutes on the host here; its
+    golden counts are checked on the GPU)."""
+    import hashlib
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rmat_counts.json")
+    rec = json.load(open(path))["rmat16"]
+    g = vsgm.Graph.rmat(16, rec["edge_factor"], *rec["abc"], seed=rec["seed"])
+    rp, cl = g.csr()
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(rp, dtype=np.uint64).tobytes())
+    h.update(np.ascontiguousarray(cl, dtype=np.uint32).tobytes())
+    assert h.hexdigest() == rec["csr_sha256"]
+    qs = {"kc3": vsgm.Query.clique(3), "kc4": vsgm.Query.clique(4), "p2": vsgm.Query.preset(2)}
+    text = {k: vsgm.Plan.optimize(q).serialize() for k, q in qs.items()}
+    assert oracle.count_plan(rp, cl, text["kc3"]) == rec["counts"]["kc3"]
+    hr, hc = g.relabel_by_degree()[0].csr()
+    for k in qs:
+        assert oracle.count_plan(hr, hc, text[k]) == rec["counts"][k], k
